@@ -1,0 +1,181 @@
+"""ctypes wrapper of the C restatement in oracle/ugks_oracle.c.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the cpu_baseline leg of bench.py, as the checker of the HIP library.  The
+product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_SRC = os.path.join(_HERE, "ugks_oracle.c")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (OpenMP, -O2, no fast-math)."""
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(_SRC):
+        cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-fno-fast-math", "-ffp-contract=off",
+               "-o", _LIB_PATH, _SRC, "-lm"]
+        subprocess.check_call(cmd, cwd=_HERE)
+    return _LIB_PATH
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nu", C.c_int), ("nv", C.c_int), ("ck", C.c_int),
+        ("macro_sweeps", C.c_int), ("micro_sweeps", C.c_int), ("mass_rescale", C.c_int), ("np_y", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("cfl", C.c_double), ("ncfl", C.c_double), ("pr", C.c_double), ("omega", C.c_double),
+        ("mu_ref", C.c_double),
+        ("wall_ux", C.c_double * 2), ("wall_tn", C.c_double * 2),
+        ("min_alpha", C.c_double), ("moment_tol", C.c_double), ("residual_tol", C.c_double),
+    ]
+
+
+_lib = None
+_D = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_create.restype = C.c_void_p
+        L.or_create.argtypes = [C.POINTER(Params), _D, _D, _D]
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_nthreads.argtypes = [C.c_int]
+        L.or_nthreads.restype = C.c_int
+        L.or_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.or_set_state.argtypes = [C.c_void_p, _D, _D, _D]
+        L.or_get_state.argtypes = [C.c_void_p, _D, _D, _D]
+        L.or_operator.argtypes = [C.c_void_p]
+        L.or_get_operator.argtypes = [C.c_void_p] + [_D] * 12
+        L.or_get_norms.argtypes = [C.c_void_p, _D]
+        L.or_implicit_update.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+        L.or_iteration.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+        L.or_observables.argtypes = [C.c_void_p, _D, _D]
+        L.or_get_implicit.argtypes = [C.c_void_p, _D, _D, _D, _D]
+        L.or_weno_coefficients.argtypes = [_D, _D, _D]
+        L.or_cell_faces9.argtypes = [_D, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, _D, _D]
+        L.or_closure_solve.argtypes = [_D, C.c_int, C.c_int, _D, _D, _D, _D, _D, _D, _D, C.POINTER(C.c_int)]
+        L.or_make_mke.argtypes = [_D, C.c_int, C.c_double, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def make_params(case, np_y: int = 1) -> Params:
+    p = Params()
+    p.np_y = int(np_y)
+    p.nx, p.ny, p.nu, p.nv, p.ck = case.nx, case.ny, case.nu, case.nv, case.ck
+    p.macro_sweeps, p.micro_sweeps, p.mass_rescale = case.macro_sweeps, case.micro_sweeps, int(case.mass_rescale)
+    p.dx, p.dy = case.dx, case.dy
+    p.cfl, p.ncfl, p.pr, p.omega, p.mu_ref = case.cfl, case.ncfl, case.pr, case.omega, case.mu_ref
+    p.wall_ux[0], p.wall_ux[1] = case.wall_ux
+    p.wall_tn[0], p.wall_tn[1] = case.wall_tn
+    p.min_alpha, p.moment_tol, p.residual_tol = case.min_alpha, case.moment_tol, case.residual_tol
+    return p
+
+
+class OracleSolver:
+    """One oracle instance of the pseudo-time step on a Couette case."""
+
+    def __init__(self, case, u, v, w, threads: int = 0, np_y: int = 1):
+        self.case = case
+        self.nvel = case.nu * case.nv
+        self.ncell = case.nx * case.ny
+        self._u = np.ascontiguousarray(u, dtype=np.float64)
+        self._v = np.ascontiguousarray(v, dtype=np.float64)
+        self._w = np.ascontiguousarray(w, dtype=np.float64)
+        L = lib()
+        if threads:
+            L.or_nthreads(int(threads))
+        self._p = make_params(case, np_y)
+        self.h = L.or_create(C.byref(self._p), ptr(self._u), ptr(self._v), ptr(self._w))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().or_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def cold_start(self, rho=1.0, ux=0.0, tnative=2.0):
+        st = lib().or_cold_start(self.h, rho, ux, tnative)
+        if st:
+            raise RuntimeError(f"oracle cold start failed ({st})")
+
+    def set_state(self, W, h, b):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        lib().or_set_state(self.h, ptr(W), ptr(h), ptr(b))
+
+    def state(self):
+        W = np.zeros((self.ncell, 4))
+        h = np.zeros((self.ncell, self.nvel))
+        b = np.zeros((self.ncell, self.nvel))
+        lib().or_get_state(self.h, ptr(W), ptr(h), ptr(b))
+        return W, h, b
+
+    def operator(self):
+        st = lib().or_operator(self.h)
+        if st:
+            raise RuntimeError(f"oracle operator failed ({st})")
+
+    def operator_outputs(self):
+        nc, n, nx, ny = self.ncell, self.nvel, self.case.nx, self.case.ny
+        out = dict(
+            dt=np.zeros(1), rate_w=np.zeros((nc, 4)), stream_h=np.zeros((nc, n)), stream_b=np.zeros((nc, n)),
+            coll_h=np.zeros((nc, n)), coll_b=np.zeros((nc, n)), xflux_h=np.zeros((nc, n)),
+            xflux_b=np.zeros((nc, n)), yflux_h=np.zeros(((ny + 1) * nx, n)), yflux_b=np.zeros(((ny + 1) * nx, n)),
+            xflux_w=np.zeros((nc, 4)), yflux_w=np.zeros(((ny + 1) * nx, 4)))
+        keys = ["dt", "rate_w", "stream_h", "stream_b", "coll_h", "coll_b", "xflux_h", "xflux_b", "yflux_h",
+                "yflux_b", "xflux_w", "yflux_w"]
+        lib().or_get_operator(self.h, *[ptr(out[k]) for k in keys])
+        out["dt"] = float(out["dt"][0])
+        return out
+
+    def norms(self):
+        o = np.zeros(24)
+        lib().or_get_norms(self.h, ptr(o))
+        return o
+
+    def implicit_update(self):
+        acc = C.c_int(0)
+        d = np.zeros(16)
+        st = lib().or_implicit_update(self.h, C.byref(acc), ptr(d))
+        return st, bool(acc.value), d
+
+    def iteration(self):
+        acc = C.c_int(0)
+        d = np.zeros(16)
+        st = lib().or_iteration(self.h, C.byref(acc), ptr(d))
+        return st, bool(acc.value), d
+
+    def implicit_fields(self):
+        nc, n = self.ncell, self.nvel
+        dW = np.zeros((nc, 4)); wt = np.zeros((nc, 4)); dh = np.zeros((nc, n)); db = np.zeros((nc, n))
+        lib().or_get_implicit(self.h, ptr(dW), ptr(wt), ptr(dh), ptr(db))
+        return dW, wt, dh, db
+
+    def observables(self):
+        f = np.zeros((self.ncell, 10))
+        w = np.zeros(16)
+        lib().or_observables(self.h, ptr(f), ptr(w))
+        return f, w

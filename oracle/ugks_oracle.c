@@ -1,0 +1,2062 @@
+/*
+ * CPU restatement (oracle) of the UGKS2D HighOrderImplicit pseudo-time step of
+ * Sherlmess/UGKS2D_high_order.  TEST INFRASTRUCTURE ONLY: used by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker
+ * of the HIP library; nothing in the product links or calls it.
+ *
+ * The structure deliberately follows the reference routine by routine, in
+ * plain loops over the velocity index, so that each function can be read
+ * against the Fortran line ranges cited in its comment.  Reference paths are
+ * relative to the reference repository root.
+ *
+ * Each part is pinned by the tests in tests/test_oracle_*.py (reference-held
+ * values, closed forms and invariants).
+ */
+#include "ugks_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define GATE 2.0e-12
+#define NEWTON_TOL 5.0e-13
+#define MICRO_MARGIN (64.0 * DBL_EPSILON)
+#define SMV ((double)1.17549435082228750797e-38) /* tiny(0.0_real32), src/mod_Globaldata.f90:26 */
+
+static const double PI_D = 3.14159265358979323846264338327950288;
+
+enum { OR_OK = 0, OR_NONFINITE = 1, OR_GATE = 2, OR_INADMISSIBLE = 3, OR_POLICY = 4, OR_NEWTON = 5 };
+
+/* ------------------------------------------------------------------------ */
+/* solver state                                                               */
+/* ------------------------------------------------------------------------ */
+struct or_solver {
+    or_params p;
+    int nvel, ncell;
+    double *u, *v, *w;        /* velocity table */
+    double umax, vmax;        /* U_MAX, V_MAX (src/mod_InitializeFlow.f90:505-506) */
+    double gamma;             /* (ck+4)/(ck+2) (src/mod_ImportPara.f90:915) */
+    double dt;
+    double *W, *h, *b;
+    /* operator outputs */
+    double *fxw, *fyw;        /* [ny][nx][4] east face of (i,j); [ny+1][nx][4] south face of row j */
+    double *fxh, *fxb, *fyh, *fyb;
+    double *mh, *mb, *kxh, *kxb, *kyh, *kyb;   /* volume targets [ncell][nvel] */
+    double *rate_w;           /* [ncell][4] */
+    double *stream_h, *stream_b, *coll_h, *coll_b, *peq_h, *peq_b;
+    double norms[24];
+    int ref_init;
+    double reference_h, reference_b;
+    /* implicit */
+    double *dW;               /* [ny][nx+2][4], x halo at columns 0 and nx+1 */
+    double *dh, *db;          /* [ny][nx+2][nvel] */
+    double *sdW, *sdh, *sdb;  /* values at the last halo exchange (y-block halos) */
+    double *w_tilde, *tau_tilde, *q_frozen;
+    double *eq_h, *eq_b, *low_h, *low_b;
+    double M0;
+    int have_M0;
+    double last_diag[16];
+};
+
+static inline int cidx(const or_solver *s, int i, int j) { return j * s->p.nx + i; }
+static inline int ip(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+
+/* ------------------------------------------------------------------------ */
+/* velocity-space moments and primitive maps                                  */
+/* ------------------------------------------------------------------------ */
+
+/* moment_2d, src/mod_CorrectedWF2DClosure.f90:25-44 */
+static int moment_2d(int n, const double *h, const double *b, const double *u, const double *v,
+                     const double *w, double m[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        m[0] += w[k] * h[k];
+        m[1] += w[k] * u[k] * h[k];
+        m[2] += w[k] * v[k] * h[k];
+        m[3] += 0.5 * w[k] * ((u[k] * u[k] + v[k] * v[k]) * h[k] + b[k]);
+    }
+    return isfinite(m[0]) && isfinite(m[1]) && isfinite(m[2]) && isfinite(m[3]);
+}
+
+/* primitive (theta_gas = p/rho), src/mod_S2D1.f90:69-81 */
+static int s2_primitive(const double w[4], int ck, double *rho, double *ux, double *vy,
+                        double *theta, double *p) {
+    *rho = w[0];
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || w[0] <= 0.0) return 0;
+    *ux = w[1] / w[0];
+    *vy = w[2] / w[0];
+    double eint = w[3] - 0.5 * w[0] * ((*ux) * (*ux) + (*vy) * (*vy));
+    if (eint <= 0.0) return 0;
+    *p = (2.0 / (double)(ck + 2)) * eint;
+    *theta = *p / w[0];
+    return isfinite(*theta) && *theta > 0.0;
+}
+
+/* heat, src/mod_S2D1.f90:83-97 */
+static int s2_heat(const double w[4], int ck, int n, const double *h, const double *b, const double *u,
+                   const double *v, const double *wt, double q[2]) {
+    double rho, ux, vy, th, p;
+    q[0] = q[1] = 0.0;
+    if (!s2_primitive(w, ck, &rho, &ux, &vy, &th, &p)) return 0;
+    for (int k = 0; k < n; ++k) {
+        double cx = u[k] - ux, cy = v[k] - vy, c2 = cx * cx + cy * cy;
+        q[0] += 0.5 * wt[k] * cx * (c2 * h[k] + b[k]);
+        q[1] += 0.5 * wt[k] * cy * (c2 * h[k] + b[k]);
+    }
+    return isfinite(q[0]) && isfinite(q[1]);
+}
+
+/* heat of the difference h-a, b-c without temporaries */
+static int s2_heat_diff(const double w[4], int ck, int n, const double *h, const double *a, const double *b,
+                        const double *c, const double *u, const double *v, const double *wt, double q[2]) {
+    double rho, ux, vy, th, p;
+    q[0] = q[1] = 0.0;
+    if (!s2_primitive(w, ck, &rho, &ux, &vy, &th, &p)) return 0;
+    for (int k = 0; k < n; ++k) {
+        double cx = u[k] - ux, cy = v[k] - vy, c2 = cx * cx + cy * cy;
+        double hh = h[k] - a[k], bb = b[k] - c[k];
+        q[0] += 0.5 * wt[k] * cx * (c2 * hh + bb);
+        q[1] += 0.5 * wt[k] * cy * (c2 * hh + bb);
+    }
+    return isfinite(q[0]) && isfinite(q[1]);
+}
+
+/* primitive_from_conserved_2d (temp = T_native), src/mod_CorrectedWF2DEquilibrium.f90:25-42 */
+static int prim_from_conserved(const double w[4], int ck, double *rho, double *u, double *v, double *temp) {
+    *rho = *u = *v = *temp = 0.0;
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3]))) return 0;
+    *rho = w[0];
+    if (*rho <= 0.0) return 0;
+    *u = w[1] / *rho;
+    *v = w[2] / *rho;
+    double kin = 0.5 * ((*u) * (*u) + (*v) * (*v));
+    double eint = w[3] / *rho - kin;
+    *temp = 4.0 * eint / (double)(ck + 2);
+    return isfinite(*u) && isfinite(*v) && isfinite(*temp) && *temp > 0.0;
+}
+
+/* analytic_maxwellian_2d, src/mod_CorrectedWF2DEquilibrium.f90:44-65 */
+static int analytic_maxwellian(const double w[4], int ck, int n, const double *u, const double *v,
+                               double *mh, double *mb) {
+    double rho, uu, vv, temp;
+    if (!prim_from_conserved(w, ck, &rho, &uu, &vv, &temp)) return 0;
+    double lambda = 1.0 / temp;
+    for (int k = 0; k < n; ++k) {
+        double c2 = (u[k] - uu) * (u[k] - uu) + (v[k] - vv) * (v[k] - vv);
+        mh[k] = rho * lambda / PI_D * exp(-lambda * c2);
+        mb[k] = 0.5 * (double)ck * temp * mh[k];
+    }
+    return 1;
+}
+
+/* solve_dense_4x4 (partial pivoting, column-major matrix(4,4)),
+   src/mod_CorrectedWF2DClosure.f90:631-661 */
+static int solve_dense_4x4(const double m[16], const double rhs[4], double x[4]) {
+    double a[4][4], b[4];
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) a[r][c] = m[c * 4 + r];
+        b[r] = rhs[r];
+        x[r] = 0.0;
+    }
+    for (int p = 0; p < 3; ++p) {
+        int row = p;
+        double best = fabs(a[p][p]);
+        for (int r = p + 1; r < 4; ++r)
+            if (fabs(a[r][p]) > best) { best = fabs(a[r][p]); row = r; }
+        if (fabs(a[row][p]) <= 100.0 * DBL_MIN || !isfinite(a[row][p])) return 0;
+        if (row != p) {
+            for (int c = 0; c < 4; ++c) { double t = a[p][c]; a[p][c] = a[row][c]; a[row][c] = t; }
+            double t = b[p]; b[p] = b[row]; b[row] = t;
+        }
+        for (int r = p + 1; r < 4; ++r) {
+            double f = a[r][p] / a[p][p];
+            for (int c = p; c < 4; ++c) a[r][c] = a[r][c] - f * a[p][c];
+            b[r] = b[r] - f * b[p];
+        }
+    }
+    if (fabs(a[3][3]) <= 100.0 * DBL_MIN || !isfinite(a[3][3])) return 0;
+    for (int r = 3; r >= 0; --r) {
+        double pv = b[r];
+        if (r < 3) {
+            double s = 0.0;
+            for (int c = r + 1; c < 4; ++c) s += a[r][c] * x[c];
+            pv = pv - s;
+        }
+        x[r] = pv / a[r][r];
+    }
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+/* SolveDense4 (Gauss-Jordan, column-major), src/mod_ImplicitCommon.f90:84-113 */
+static int solve_dense4_gj(const double m[16], const double rhs[4], double x[4]) {
+    double a[4][5];
+    double scale = 0.0;
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = m[c * 4 + r];
+            if (fabs(a[r][c]) > scale) scale = fabs(a[r][c]);
+        }
+        a[r][4] = rhs[r];
+    }
+    if (scale < DBL_MIN) scale = DBL_MIN;
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
+        if (fabs(a[piv][col]) <= 1000.0 * DBL_EPSILON * scale) return 0;
+        if (piv != col)
+            for (int c = 0; c < 5; ++c) { double t = a[col][c]; a[col][c] = a[piv][c]; a[piv][c] = t; }
+        double d = a[col][col];
+        for (int c = 0; c < 5; ++c) a[col][c] = a[col][c] / d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            double f = a[r][col];
+            for (int c = 0; c < 5; ++c) a[r][c] = a[r][c] - f * a[col][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r) x[r] = a[r][4];
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* discrete conservative equilibrium (Newton closure)                         */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    double *base;
+    double *qh, *qb, *qht, *qbt;   /* qht[a*n+k] */
+    double *nqh, *nqb, *nqht, *nqbt;
+    double theta[4], jac[16];
+    int iters;
+} closure_ws;
+
+static void closure_ws_init(closure_ws *c, int n) {
+    c->n = n;
+    c->base = (double *)malloc(sizeof(double) * n * 20);
+    c->qh = c->base;
+    c->qb = c->qh + n;
+    c->qht = c->qb + n;
+    c->qbt = c->qht + 4 * n;
+    c->nqh = c->qbt + 4 * n;
+    c->nqb = c->nqh + n;
+    c->nqht = c->nqb + n;
+    c->nqbt = c->nqht + 4 * n;
+}
+static void closure_ws_free(closure_ws *c) { free(c->base); }
+
+/* q_state_first, src/mod_CorrectedWF2DClosure.f90:394-432 */
+static int q_state_first(const double th[4], int ck, int n, const double *u, const double *v, const double *wt,
+                         double *qh, double *qb, double *qht, double *qbt, double jac[16]) {
+    if (th[0] <= 0.0 || th[3] <= 0.0) return 0;
+    double r = th[0], tw = th[3];
+    for (int k = 0; k < n; ++k) {
+        double du = u[k] - th[1], dv = v[k] - th[2], c2 = du * du + dv * dv;
+        qh[k] = r / (PI_D * tw) * exp(-c2 / tw);
+        double logd[4] = {1.0 / r, 2.0 * du / tw, 2.0 * dv / tw, -1.0 / tw + c2 / (tw * tw)};
+        for (int a = 0; a < 4; ++a) qht[a * n + k] = qh[k] * logd[a];
+        if (ck > 0) {
+            qb[k] = 0.5 * (double)ck * tw * qh[k];
+            logd[3] = logd[3] + 1.0 / tw;
+            for (int a = 0; a < 4; ++a) qbt[a * n + k] = qb[k] * logd[a];
+        } else {
+            qb[k] = 0.0;
+            for (int a = 0; a < 4; ++a) qbt[a * n + k] = 0.0;
+        }
+    }
+    for (int a = 0; a < 4; ++a)
+        if (!moment_2d(n, qht + a * n, qbt + a * n, u, v, wt, jac + 4 * a)) return 0;
+    for (int k = 0; k < n; ++k)
+        if (!isfinite(qh[k]) || !isfinite(qb[k])) return 0;
+    for (int a = 0; a < 16; ++a)
+        if (!isfinite(jac[a])) return 0;
+    return 1;
+}
+
+static double scaled_residual(const double r[4], const double t[4]) {
+    double m = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        double s = fabs(t[a]) > 1.0 ? fabs(t[a]) : 1.0;
+        double x = fabs(r[a]) / s;
+        if (x > m) m = x;
+    }
+    return m;
+}
+
+/* solve_q_first (Newton with halving line search), src/mod_CorrectedWF2DClosure.f90:330-392 */
+static int solve_q_first(closure_ws *c, const double w[4], int ck, const double *u, const double *v,
+                         const double *wt) {
+    int n = c->n;
+    double rho, uu, vv, temp, res[4], nres[4], step[4], nth[4], njac[16];
+    c->iters = 0;
+    if (!prim_from_conserved(w, ck, &rho, &uu, &vv, &temp)) return 0;
+    c->theta[0] = rho; c->theta[1] = uu; c->theta[2] = vv; c->theta[3] = temp;
+    if (!q_state_first(c->theta, ck, n, u, v, wt, c->qh, c->qb, c->qht, c->qbt, c->jac)) return 0;
+    if (!moment_2d(n, c->qh, c->qb, u, v, wt, res)) return 0;
+    for (int a = 0; a < 4; ++a) res[a] -= w[a];
+    double norm_old = scaled_residual(res, w);
+    for (int it = 1; it <= 12; ++it) {
+        if (norm_old <= NEWTON_TOL) break;
+        double mres[4] = {-res[0], -res[1], -res[2], -res[3]};
+        if (!solve_dense_4x4(c->jac, mres, step)) return 0;
+        c->iters = it;
+        double alpha = 1.0;
+        int accepted = 0;
+        while (alpha >= 9.5367431640625e-07) { /* 2**(-20) */
+            for (int a = 0; a < 4; ++a) nth[a] = c->theta[a] + alpha * step[a];
+            if (nth[0] > 0.0 && nth[3] > 0.0 && isfinite(nth[0]) && isfinite(nth[1]) && isfinite(nth[2]) &&
+                isfinite(nth[3])) {
+                if (q_state_first(nth, ck, n, u, v, wt, c->nqh, c->nqb, c->nqht, c->nqbt, njac) &&
+                    moment_2d(n, c->nqh, c->nqb, u, v, wt, nres)) {
+                    for (int a = 0; a < 4; ++a) nres[a] -= w[a];
+                    double norm_new = scaled_residual(nres, w);
+                    if (norm_new < norm_old || norm_new <= NEWTON_TOL) {
+                        accepted = 1;
+                        memcpy(c->theta, nth, sizeof nth);
+                        memcpy(c->jac, njac, sizeof njac);
+                        memcpy(res, nres, sizeof nres);
+                        norm_old = norm_new;
+                        double *t;
+                        t = c->qh; c->qh = c->nqh; c->nqh = t;
+                        t = c->qb; c->qb = c->nqb; c->nqb = t;
+                        t = c->qht; c->qht = c->nqht; c->nqht = t;
+                        t = c->qbt; c->qbt = c->nqbt; c->nqbt = t;
+                    }
+                }
+                if (accepted) break;
+            }
+            alpha = 0.5 * alpha;
+        }
+        if (!accepted) return 0;
+    }
+    if (!(norm_old <= NEWTON_TOL)) return 0;
+    for (int k = 0; k < n; ++k)
+        if (!(c->qh[k] > 0.0) || !(c->qb[k] > 0.0) || !isfinite(c->qh[k]) || !isfinite(c->qb[k])) return 0;
+    return 1;
+}
+
+/* closure_first_direction, src/mod_CorrectedWF2DClosure.f90:104-121 */
+static int closure_direction(const closure_ws *c, const double m[4], double *dh, double *db) {
+    double td[4];
+    int n = c->n;
+    if (!solve_dense_4x4(c->jac, m, td)) return 0;
+    for (int k = 0; k < n; ++k) { dh[k] = 0.0; db[k] = 0.0; }
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < n; ++k) {
+            dh[k] += c->qht[a * n + k] * td[a];
+            db[k] += c->qbt[a * n + k] * td[a];
+        }
+    return 1;
+}
+
+/* closure_first_project, src/mod_CorrectedWF2DClosure.f90:123-144 (in place) */
+static int closure_project(const closure_ws *c, double *zh, double *zb, const double *u, const double *v,
+                           const double *wt, double *tmph, double *tmpb) {
+    double m[4];
+    if (!moment_2d(c->n, zh, zb, u, v, wt, m)) return 0;
+    if (!closure_direction(c, m, tmph, tmpb)) return 0;
+    for (int k = 0; k < c->n; ++k) { zh[k] -= tmph[k]; zb[k] -= tmpb[k]; }
+    return 1;
+}
+
+/* make_mke_prepared, src/mod_S2D1.f90:110-151.  ws holds 2*n scratch. */
+static int make_mke_prepared(const closure_ws *c, const double w[4], int ck, double pr, const double *u,
+                             const double *v, const double *wt, double *mh, double *mb, double *kxh,
+                             double *kxb, double *kyh, double *kyb, double *eah, double *eab, double kappa[4],
+                             double *residual, double *scratch) {
+    int n = c->n;
+    double rho, ux, vy, th, p, mom[4], q[2];
+    *residual = 0.0;
+    if (!s2_primitive(w, ck, &rho, &ux, &vy, &th, &p)) return 0;
+    if (eah && !analytic_maxwellian(w, ck, n, u, v, eah, eab)) return 0;
+    memcpy(mh, c->qh, sizeof(double) * n);
+    memcpy(mb, c->qb, sizeof(double) * n);
+    double lambda = 0.5 / th;
+    double coeff = 0.8 * (1.0 - pr) * lambda * lambda / rho;
+    for (int k = 0; k < n; ++k) {
+        double cx = u[k] - ux, cy = v[k] - vy, c2 = cx * cx + cy * cy;
+        double shapeh = 2.0 * lambda * c2 - 4.0;
+        double shapeb = shapeh + 2.0 / (double)ck;
+        kxh[k] = coeff * cx * shapeh * mh[k];
+        kxb[k] = coeff * cx * shapeb * mb[k];
+        kyh[k] = coeff * cy * shapeh * mh[k];
+        kyb[k] = coeff * cy * shapeb * mb[k];
+    }
+    if (!closure_project(c, kxh, kxb, u, v, wt, scratch, scratch + n)) return 0;
+    if (!closure_project(c, kyh, kyb, u, v, wt, scratch, scratch + n)) return 0;
+    if (kappa) {
+        if (!s2_heat(w, ck, n, kxh, kxb, u, v, wt, q)) return 0;
+        kappa[0] = q[0]; kappa[1] = q[1];
+        if (!s2_heat(w, ck, n, kyh, kyb, u, v, wt, q)) return 0;
+        kappa[2] = q[0]; kappa[3] = q[1];
+    }
+    if (!moment_2d(n, mh, mb, u, v, wt, mom)) return 0;
+    double wmax = 1.0, r = 0.0;
+    for (int a = 0; a < 4; ++a) if (fabs(w[a]) > wmax) wmax = fabs(w[a]);
+    for (int a = 0; a < 4; ++a) if (fabs(mom[a] - w[a]) > r) r = fabs(mom[a] - w[a]);
+    r /= wmax;
+    if (!moment_2d(n, kxh, kxb, u, v, wt, mom)) return 0;
+    for (int a = 0; a < 4; ++a) if (fabs(mom[a]) > r) r = fabs(mom[a]);
+    if (!moment_2d(n, kyh, kyb, u, v, wt, mom)) return 0;
+    for (int a = 0; a < 4; ++a) if (fabs(mom[a]) > r) r = fabs(mom[a]);
+    *residual = r;
+    return r <= GATE;
+}
+
+/* m_direction_prepared, src/mod_S2D1.f90:164-178 */
+static int m_direction_prepared(const closure_ws *c, const double dw[4], const double *u, const double *v,
+                                const double *wt, double *dh, double *db) {
+    double mom[4];
+    if (!closure_direction(c, dw, dh, db)) return 0;
+    if (!moment_2d(c->n, dh, db, u, v, wt, mom)) return 0;
+    double s = 1.0, r = 0.0;
+    for (int a = 0; a < 4; ++a) if (fabs(dw[a]) > s) s = fabs(dw[a]);
+    for (int a = 0; a < 4; ++a) if (fabs(mom[a] - dw[a]) > r) r = fabs(mom[a] - dw[a]);
+    return r / s <= GATE;
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5-AO reconstruction                                                    */
+/* ------------------------------------------------------------------------ */
+static const double WENO_EPS = 1.0e-8;
+static const double WENO_GAMMA[4] = {0.01125, 0.1275, 0.01125, 0.85};
+static double GNODE[3];
+static const double GW[3] = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+static double CMAP[4][5][5];          /* [candidate][sample][degree] */
+static int maps_ready = 0;
+
+/* Gaussian elimination with partial pivoting, src/mod_S2D1Reconstruction.f90:147-169 */
+static void solve_small(int n, const double *a_in, const double *b_in, double *x) {
+    double a[5][5], r[5];
+    for (int i = 0; i < n; ++i) { for (int j = 0; j < n; ++j) a[i][j] = a_in[i * n + j]; r[i] = b_in[i]; }
+    for (int p = 0; p < n - 1; ++p) {
+        int m = p;
+        for (int i = p + 1; i < n; ++i) if (fabs(a[i][p]) > fabs(a[m][p])) m = i;
+        if (m != p) {
+            for (int j = 0; j < n; ++j) { double t = a[p][j]; a[p][j] = a[m][j]; a[m][j] = t; }
+            double t = r[p]; r[p] = r[m]; r[m] = t;
+        }
+        for (int i = p + 1; i < n; ++i) {
+            double f = a[i][p] / a[p][p];
+            for (int j = p; j < n; ++j) a[i][j] = a[i][j] - f * a[p][j];
+            r[i] = r[i] - f * r[p];
+        }
+    }
+    x[n - 1] = r[n - 1] / a[n - 1][n - 1];
+    for (int i = n - 2; i >= 0; --i) {
+        double s = 0.0;
+        for (int j = i + 1; j < n; ++j) s += a[i][j] * x[j];
+        x[i] = (r[i] - s) / a[i][i];
+    }
+}
+
+/* polynomial coefficients whose cell averages over the given offsets are the
+   samples: map[s][d] (build_map, src/mod_S2D1Reconstruction.f90:81-109) */
+static void build_map(int n, const int *off, double map[5][5]) {
+    double a[25], rhs[5], x[5];
+    for (int r = 0; r < n; ++r)
+        for (int d = 0; d < n; ++d)
+            a[r * n + d] = (pow(off[r] + 0.5, d + 1) - pow(off[r] - 0.5, d + 1)) / (double)(d + 1);
+    for (int s = 0; s < 5; ++s) for (int d = 0; d < 5; ++d) map[s][d] = 0.0;
+    for (int c = 0; c < n; ++c) {
+        for (int r = 0; r < n; ++r) rhs[r] = (r == c) ? 1.0 : 0.0;
+        solve_small(n, a, rhs, x);
+        for (int d = 0; d < n; ++d) map[c][d] = x[d];
+    }
+}
+
+static void init_maps(void) {
+    if (maps_ready) return;
+    GNODE[0] = -0.5 * sqrt(3.0 / 5.0);
+    GNODE[1] = 0.0;
+    GNODE[2] = 0.5 * sqrt(3.0 / 5.0);
+    for (int c = 0; c < 4; ++c)
+        for (int s = 0; s < 5; ++s)
+            for (int d = 0; d < 5; ++d) CMAP[c][s][d] = 0.0;
+    /* init_maps, src/mod_S2D1Reconstruction.f90:57-79 */
+    int o1[3] = {-2, -1, 0}, o2[3] = {-1, 0, 1}, o3[3] = {0, 1, 2}, o4[5] = {-2, -1, 0, 1, 2};
+    double m[5][5];
+    build_map(3, o1, m);
+    for (int s = 0; s < 3; ++s) for (int d = 0; d < 3; ++d) CMAP[0][s + 0][d] = m[s][d];
+    build_map(3, o2, m);
+    for (int s = 0; s < 3; ++s) for (int d = 0; d < 3; ++d) CMAP[1][s + 1][d] = m[s][d];
+    build_map(3, o3, m);
+    for (int s = 0; s < 3; ++s) for (int d = 0; d < 3; ++d) CMAP[2][s + 2][d] = m[s][d];
+    build_map(5, o4, m);
+    for (int s = 0; s < 5; ++s) for (int d = 0; d < 5; ++d) CMAP[3][s][d] = m[s][d];
+    maps_ready = 1;
+}
+
+/* one-sided quartic through five rows for a near-wall cell; samples are rows
+   first..first+4 and the cell sits at offset 0 (wall_coeff / wallmap,
+   src/mod_S2D1Reconstruction.f90:64-70,113-136) */
+static void wall_map_for(int first, int j, double map[5][5]) {
+    int off[5];
+    for (int s = 0; s < 5; ++s) off[s] = first + s - j;
+    build_map(5, off, map);
+}
+
+static void wall_eval(const double map[5][5], const double q[5], double xi, double *val, double *der) {
+    double c[5];
+    for (int d = 0; d < 5; ++d) {
+        c[d] = 0.0;
+        for (int s = 0; s < 5; ++s) c[d] += map[s][d] * q[s];
+    }
+    double vv = c[4];
+    for (int d = 3; d >= 0; --d) vv = vv * xi + c[d];
+    double dd = 4.0 * c[4];
+    for (int d = 3; d >= 1; --d) dd = dd * xi + (double)d * c[d];
+    *val = vv;
+    if (der) *der = dd;
+}
+
+/* weno_coefficients, src/mod_S2D1Reconstruction.f90:171-204 */
+static void weno_coef(const double q[5], double fw[4], double coef[4][5]) {
+    double beta[4], alpha[4], omega[4];
+    double t;
+    t = q[0] - 2 * q[1] + q[2]; beta[0] = 13.0 / 12.0 * t * t;
+    t = q[0] - 4 * q[1] + 3 * q[2]; beta[0] += 0.25 * t * t;
+    t = q[1] - 2 * q[2] + q[3]; beta[1] = 13.0 / 12.0 * t * t;
+    t = q[1] - q[3]; beta[1] += 0.25 * t * t;
+    t = q[2] - 2 * q[3] + q[4]; beta[2] = 13.0 / 12.0 * t * t;
+    t = 3 * q[2] - 4 * q[3] + q[4]; beta[2] += 0.25 * t * t;
+    beta[3] = (231153 * q[2] * q[2] + 104963 * q[1] * q[1] + 6908 * q[0] * q[0] - 38947 * q[0] * q[3] +
+               104963 * q[3] * q[3] + q[1] * (-51001 * q[0] + 179098 * q[3] - 38947 * q[4]) -
+               3 * q[2] * (99692 * q[1] - 22641 * q[0] + 99692 * q[3] - 22641 * q[4]) + 8209 * q[0] * q[4] -
+               51001 * q[3] * q[4] + 6908 * q[4] * q[4]) /
+              5040.0;
+    double tau = (fabs(beta[3] - beta[0]) + fabs(beta[3] - beta[1]) + fabs(beta[3] - beta[2])) / 3.0;
+    double sa = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        double r = tau / (beta[c] + WENO_EPS);
+        alpha[c] = WENO_GAMMA[c] * (1.0 + r * r);
+        sa += alpha[c];
+    }
+    for (int c = 0; c < 4; ++c) omega[c] = alpha[c] / sa;
+    fw[3] = omega[3] / WENO_GAMMA[3];
+    for (int c = 0; c < 3; ++c) fw[c] = omega[c] - fw[3] * WENO_GAMMA[c];
+    for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 5; ++d) coef[c][d] = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int s = c; s < c + 3; ++s)
+            for (int d = 0; d < 3; ++d) coef[c][d] += CMAP[c][s][d] * q[s];
+    for (int s = 0; s < 5; ++s)
+        for (int d = 0; d < 5; ++d) coef[3][d] += CMAP[3][s][d] * q[s];
+}
+
+/* prepared_eval, src/mod_S2D1Reconstruction.f90:238-253 */
+static void weno_eval(const double fw[4], const double coef[4][5], double xi, double *val, double *der) {
+    double pw[5], dp[5];
+    pw[0] = 1.0; dp[0] = 0.0;
+    for (int d = 1; d < 5; ++d) { pw[d] = pw[d - 1] * xi; dp[d] = (double)d * pow(xi, d - 1); }
+    double vv = 0.0, dd = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        double cv = 0.0, cd = 0.0;
+        for (int d = 0; d < 5; ++d) { cv += coef[c][d] * pw[d]; cd += coef[c][d] * dp[d]; }
+        vv += fw[c] * cv;
+        dd += fw[c] * cd;
+    }
+    *val = vv;
+    if (der) *der = dd;
+}
+
+/* companion_eval3: value and derivative of q at the three Gauss nodes, plus
+   the companion interpolated with q's frozen nonlinear weights,
+   src/mod_S2D1Reconstruction.f90:298-326 */
+static void companion_eval3(const double q[5], const double comp[5], double val[3], double der[3],
+                            double cval[3]) {
+    double fw[4], coef[4][5], ccoef[4][5], fdummy[4];
+    weno_coef(q, fw, coef);
+    weno_coef(comp, fdummy, ccoef);   /* linear candidate coefficients of the companion */
+    for (int a = 0; a < 3; ++a) {
+        weno_eval(fw, coef, GNODE[a], &val[a], &der[a]);
+        weno_eval(fw, ccoef, GNODE[a], &cval[a], NULL);
+    }
+}
+
+void or_weno_coefficients(const double q[5], double fw[4], double coef[20]) {
+    init_maps();
+    double c[4][5];
+    weno_coef(q, fw, c);
+    for (int a = 0; a < 4; ++a) for (int d = 0; d < 5; ++d) coef[a * 5 + d] = c[a][d];
+}
+
+static inline int near_wall(int j, int ny) { return j <= 2 || j >= ny - 3; }
+static inline int wall_first(int j, int ny) { return j <= 2 ? 0 : ny - 5; }
+
+/* Cell reconstruction of one scalar plane field[j*nx+i] (x periodic, walls in
+   y): faces 0..3 (west, east, south, north) at three Gauss nodes with their
+   tangential (dt) and normal (dn) derivatives per unit cell width, and the
+   3x3 volume nodes vol[a*3+b] (a: x node, b: y node).  Interior cells follow
+   s2d1_cell_faces9 (src/mod_S2D1Reconstruction.f90:379-413), near-wall cells
+   s2d1_cell_faces9_wall (:415-460). */
+static void cell_faces9(const double *F, int nx, int ny, int i, int j, double face[4][3], double dn[4][3],
+                        double dt[4][3], double vol[3][3]) {
+    double q[5], fw[4], coef[4][5];
+    double xv[5][2], xd[5][2], rows[5][3], yv[5][2], yd[5][2];
+    if (near_wall(j, ny)) {
+        int first = wall_first(j, ny);
+        double wm[5][5];
+        wall_map_for(first, j, wm);
+        for (int r = 0; r < 5; ++r) {
+            int row = first + r;
+            for (int s = 0; s < 5; ++s) q[s] = F[row * nx + ip(i + s - 2, nx)];
+            weno_coef(q, fw, coef);
+            weno_eval(fw, coef, -0.5, &xv[r][0], &xd[r][0]);
+            weno_eval(fw, coef, 0.5, &xv[r][1], &xd[r][1]);
+            for (int a = 0; a < 3; ++a) weno_eval(fw, coef, GNODE[a], &rows[r][a], NULL);
+        }
+        for (int r = 0; r < 5; ++r) {
+            int col = ip(i + r - 2, nx);
+            for (int s = 0; s < 5; ++s) q[s] = F[(first + s) * nx + col];
+            wall_eval(wm, q, -0.5, &yv[r][0], &yd[r][0]);
+            wall_eval(wm, q, 0.5, &yv[r][1], &yd[r][1]);
+        }
+        for (int s = 0; s < 2; ++s) {
+            double tq[5], tc[5];
+            for (int r = 0; r < 5; ++r) { tq[r] = xv[r][s]; tc[r] = xd[r][s]; }
+            for (int a = 0; a < 3; ++a) {
+                wall_eval(wm, tq, GNODE[a], &face[s][a], &dt[s][a]);
+                wall_eval(wm, tc, GNODE[a], &dn[s][a], NULL);
+            }
+            for (int r = 0; r < 5; ++r) { tq[r] = yv[r][s]; tc[r] = yd[r][s]; }
+            companion_eval3(tq, tc, face[s + 2], dt[s + 2], dn[s + 2]);
+        }
+        for (int a = 0; a < 3; ++a) {
+            double tq[5];
+            for (int r = 0; r < 5; ++r) tq[r] = rows[r][a];
+            for (int b = 0; b < 3; ++b) wall_eval(wm, tq, GNODE[b], &vol[a][b], NULL);
+        }
+        return;
+    }
+    for (int r = 0; r < 5; ++r) {
+        int row = j + r - 2;
+        for (int s = 0; s < 5; ++s) q[s] = F[row * nx + ip(i + s - 2, nx)];
+        weno_coef(q, fw, coef);
+        weno_eval(fw, coef, -0.5, &xv[r][0], &xd[r][0]);
+        weno_eval(fw, coef, 0.5, &xv[r][1], &xd[r][1]);
+        for (int a = 0; a < 3; ++a) weno_eval(fw, coef, GNODE[a], &rows[r][a], NULL);
+    }
+    for (int r = 0; r < 5; ++r) {
+        int col = ip(i + r - 2, nx);
+        for (int s = 0; s < 5; ++s) q[s] = F[(j + s - 2) * nx + col];
+        weno_coef(q, fw, coef);
+        weno_eval(fw, coef, -0.5, &yv[r][0], &yd[r][0]);
+        weno_eval(fw, coef, 0.5, &yv[r][1], &yd[r][1]);
+    }
+    for (int s = 0; s < 2; ++s) {
+        double tq[5], tc[5];
+        for (int r = 0; r < 5; ++r) { tq[r] = xv[r][s]; tc[r] = xd[r][s]; }
+        companion_eval3(tq, tc, face[s], dt[s], dn[s]);
+        for (int r = 0; r < 5; ++r) { tq[r] = yv[r][s]; tc[r] = yd[r][s]; }
+        companion_eval3(tq, tc, face[s + 2], dt[s + 2], dn[s + 2]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        double tq[5];
+        for (int r = 0; r < 5; ++r) tq[r] = rows[r][a];
+        weno_coef(tq, fw, coef);
+        for (int b = 0; b < 3; ++b) weno_eval(fw, coef, GNODE[b], &vol[a][b], NULL);
+    }
+}
+
+int or_cell_faces9(const double *field, int nx, int ny, int i, int j, double face[12], double dn[12],
+                   double dt[12], double vol[9]) {
+    init_maps();
+    double f[4][3], n[4][3], t[4][3], v[3][3];
+    cell_faces9(field, nx, ny, i, j, f, n, t, v);
+    memcpy(face, f, sizeof f);
+    memcpy(dn, n, sizeof n);
+    memcpy(dt, t, sizeof t);
+    memcpy(vol, v, sizeof v);
+    return 0;
+}
+
+/* limit_cell_outputs: positivity limiter toward the cell means, then scaling
+   of the derivatives to physical units, src/mod_S2D1.f90:315-345 */
+static void limit_cell(double hm, double bm, double dx, double dy, double hf[4][3], double bf[4][3],
+                       double hn[4][3], double ht[4][3], double bn[4][3], double bt[4][3], double hv[3][3],
+                       double bv[3][3]) {
+    double theta = 1.0;
+    if (hm <= 0.0 || bm <= 0.0) {
+        theta = 0.0;
+    } else {
+        for (int s = 0; s < 4; ++s)
+            for (int a = 0; a < 3; ++a) {
+                if (hf[s][a] < 0.0) { double t = (1.0 - MICRO_MARGIN) * hm / (hm - hf[s][a]); if (t < theta) theta = t; }
+                if (bf[s][a] < 0.0) { double t = (1.0 - MICRO_MARGIN) * bm / (bm - bf[s][a]); if (t < theta) theta = t; }
+            }
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) {
+                /* vol(a,c) in the reference is indexed [x node][y node] */
+                if (hv[a][c] < 0.0) { double t = (1.0 - MICRO_MARGIN) * hm / (hm - hv[a][c]); if (t < theta) theta = t; }
+                if (bv[a][c] < 0.0) { double t = (1.0 - MICRO_MARGIN) * bm / (bm - bv[a][c]); if (t < theta) theta = t; }
+            }
+        if (theta > 1.0) theta = 1.0;
+        if (theta < 0.0) theta = 0.0;
+    }
+    for (int s = 0; s < 4; ++s)
+        for (int a = 0; a < 3; ++a) {
+            hf[s][a] = hm + theta * (hf[s][a] - hm);
+            bf[s][a] = bm + theta * (bf[s][a] - bm);
+            hn[s][a] = theta * hn[s][a]; ht[s][a] = theta * ht[s][a];
+            bn[s][a] = theta * bn[s][a]; bt[s][a] = theta * bt[s][a];
+            double sn = (s < 2) ? dx : dy, st = (s < 2) ? dy : dx;
+            hn[s][a] /= sn; ht[s][a] /= st; bn[s][a] /= sn; bt[s][a] /= st;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* gas model                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* GetTau (power law), src/mod_Tools.f90:267-300; prim = (rho,U,V,lambda) */
+static double get_tau(const or_params *p, const double prim[4]) {
+    double mu = p->mu_ref * pow(1.0 / prim[3], p->omega);
+    return mu / (0.5 * prim[0] / prim[3]);
+}
+
+/* s2_tau, src/mod_S2PhysicalGas.f90:10-23 */
+static double s2_tau(const or_params *p, const double w[4]) {
+    double pr = 2.0 / (double)(p->ck + 2) * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+    double prim[4] = {w[0], w[1] / w[0], w[2] / w[0], 0.5 * w[0] / pr};
+    return get_tau(p, prim);
+}
+
+/* GetPrimary, src/mod_Tools.f90:209-235 */
+static void get_primary(double gamma, const double w[4], double prim[4]) {
+    double denom = w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0];
+    prim[0] = w[0];
+    prim[1] = w[1] / w[0];
+    prim[2] = w[2] / w[0];
+    prim[3] = 0.5 * w[0] / ((gamma - 1.0) * denom);
+}
+
+/* IsPhysicalState, src/mod_ImplicitCommon.f90:10-20 */
+static int is_physical(const double w[4]) {
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || w[0] <= 0.0) return 0;
+    double e = w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0];
+    return isfinite(e) && e > 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* UGKS face node                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    closure_ws cl;
+    double *f0h, *f0b, *fnh, *fnb, *fth, *ftb, *mh, *mb, *kxh, *kxb, *kyh, *kyb, *eah, *eab;
+    double *s0h, *s0b, *mnlh, *mnlb, *mnrh, *mnrb, *mth, *mtb, *mtimeh, *mtimeb, *tmp;
+    double *fh, *fb;
+    double *buf;
+} node_ws;
+
+static void node_ws_init(node_ws *w, int n) {
+    w->n = n;
+    closure_ws_init(&w->cl, n);
+    w->buf = (double *)malloc(sizeof(double) * n * 30);
+    double *p = w->buf;
+    double **ptr[] = {&w->f0h, &w->f0b, &w->fnh, &w->fnb, &w->fth, &w->ftb, &w->mh, &w->mb, &w->kxh,
+                      &w->kxb, &w->kyh, &w->kyb, &w->eah, &w->eab, &w->s0h, &w->s0b, &w->mnlh, &w->mnlb,
+                      &w->mnrh, &w->mnrb, &w->mth, &w->mtb, &w->mtimeh, &w->mtimeb, &w->fh, &w->fb};
+    for (size_t a = 0; a < sizeof ptr / sizeof ptr[0]; ++a) { *ptr[a] = p; p += n; }
+    w->tmp = p; /* 2n */
+}
+static void node_ws_free(node_ws *w) { closure_ws_free(&w->cl); free(w->buf); }
+
+/* ugks_node: one Gauss point of an interior face, src/mod_S2D1.f90:421-521.
+   Inputs are the left/right reconstructed values and derivatives at the node
+   (already limited and scaled), the macro slopes, the velocity table and dt;
+   output fh, fb (in ws) and fw. */
+static int ugks_node(const or_params *P, node_ws *ws, int axis, double dt, const double *hl, const double *hr,
+                     const double *bl, const double *br, const double *hnl, const double *hnr,
+                     const double *htl, const double *htr, const double *bnl, const double *bnr,
+                     const double *btl, const double *btr, const double mnl[4], const double mnr[4],
+                     const double mtg[4], const double *u, const double *v, const double *wt, double fw[4]) {
+    int n = ws->n, ck = P->ck;
+    double w0[4], q0[2], wtime[4], kap[4], res, mt[5];
+    for (int k = 0; k < n; ++k) {
+        double vn = axis == 1 ? u[k] : v[k];
+        if (vn >= 0.0) {
+            ws->f0h[k] = hl[k]; ws->f0b[k] = bl[k]; ws->fnh[k] = hnl[k]; ws->fnb[k] = bnl[k];
+            ws->fth[k] = htl[k]; ws->ftb[k] = btl[k];
+        } else {
+            ws->f0h[k] = hr[k]; ws->f0b[k] = br[k]; ws->fnh[k] = hnr[k]; ws->fnb[k] = bnr[k];
+            ws->fth[k] = htr[k]; ws->ftb[k] = btr[k];
+        }
+    }
+    if (!moment_2d(n, ws->f0h, ws->f0b, u, v, wt, w0)) return OR_NONFINITE;
+    if (!solve_q_first(&ws->cl, w0, ck, u, v, wt)) return OR_NEWTON;
+    if (!make_mke_prepared(&ws->cl, w0, ck, P->pr, u, v, wt, ws->mh, ws->mb, ws->kxh, ws->kxb, ws->kyh, ws->kyb,
+                           ws->eah, ws->eab, kap, &res, ws->tmp))
+        return OR_GATE;
+    if (!s2_heat_diff(w0, ck, n, ws->f0h, ws->eah, ws->f0b, ws->eab, u, v, wt, q0)) return OR_NONFINITE;
+    for (int k = 0; k < n; ++k) {
+        ws->s0h[k] = ws->mh[k] + ws->kxh[k] * q0[0] + ws->kyh[k] * q0[1];
+        ws->s0b[k] = ws->mb[k] + ws->kxb[k] * q0[0] + ws->kyb[k] * q0[1];
+    }
+    if (!m_direction_prepared(&ws->cl, mnl, u, v, wt, ws->mnlh, ws->mnlb)) return OR_GATE;
+    if (!m_direction_prepared(&ws->cl, mnr, u, v, wt, ws->mnrh, ws->mnrb)) return OR_GATE;
+    if (!m_direction_prepared(&ws->cl, mtg, u, v, wt, ws->mth, ws->mtb)) return OR_GATE;
+    /* time slope: wtime = -moments(vn * upwind normal direction + vt * tangential direction) */
+    double *tnh = ws->tmp, *tnb = ws->tmp + n;
+    for (int k = 0; k < n; ++k) {
+        double vn = axis == 1 ? u[k] : v[k], vt = axis == 1 ? v[k] : u[k];
+        double a = vn >= 0.0 ? ws->mnlh[k] : ws->mnrh[k];
+        double c = vn >= 0.0 ? ws->mnlb[k] : ws->mnrb[k];
+        tnh[k] = vn * a + vt * ws->mth[k];
+        tnb[k] = vn * c + vt * ws->mtb[k];
+    }
+    if (!moment_2d(n, tnh, tnb, u, v, wt, wtime)) return OR_NONFINITE;
+    for (int a = 0; a < 4; ++a) wtime[a] = -wtime[a];
+    if (!m_direction_prepared(&ws->cl, wtime, u, v, wt, ws->mtimeh, ws->mtimeb)) return OR_GATE;
+    /* variable-tau UGKS time integrals, src/mod_S2D1.f90:484-488 */
+    double tau = s2_tau(P, w0);
+    double ex = exp(-dt / tau);
+    mt[3] = tau * (1.0 - ex);
+    mt[4] = -tau * dt * ex + tau * mt[3];
+    mt[0] = dt - mt[3];
+    mt[1] = -tau * mt[0] + mt[4];
+    mt[2] = 0.5 * dt * dt - tau * mt[0];
+    for (int k = 0; k < n; ++k) {
+        double vn = axis == 1 ? u[k] : v[k], vt = axis == 1 ? v[k] : u[k];
+        double slh, slb;
+        if (vn >= 0.0) {
+            slh = vn * ws->mnlh[k] + vt * ws->mth[k];
+            slb = vn * ws->mnlb[k] + vt * ws->mtb[k];
+        } else {
+            slh = vn * ws->mnrh[k] + vt * ws->mth[k];
+            slb = vn * ws->mnrb[k] + vt * ws->mtb[k];
+        }
+        ws->fh[k] = vn * (mt[0] * ws->s0h[k] + mt[1] * slh + mt[2] * ws->mtimeh[k] + mt[3] * ws->f0h[k] -
+                          mt[4] * (vn * ws->fnh[k] + vt * ws->fth[k]));
+        ws->fb[k] = vn * (mt[0] * ws->s0b[k] + mt[1] * slb + mt[2] * ws->mtimeb[k] + mt[3] * ws->f0b[k] -
+                          mt[4] * (vn * ws->fnb[k] + vt * ws->ftb[k]));
+    }
+    if (!moment_2d(n, ws->fh, ws->fb, u, v, wt, fw)) return OR_NONFINITE;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* multiscale diffuse wall node (physical gas), src/mod_S2MultiscaleWall.f90  */
+/* ------------------------------------------------------------------------ */
+
+/* s2w2_time_coefficients, src/mod_S2MultiscaleWall.f90:18-43 */
+static void wall_time_coefficients(double dt, double tau, double mt[5]) {
+    double x = dt / tau;
+    if (x < 0.1) {
+        double t = 1.0;
+        for (int a = 0; a < 5; ++a) mt[a] = 0.0;
+        for (int n = 1; n <= 24; ++n) {
+            t = t * (-x) / (double)n;
+            mt[3] = mt[3] - tau * t;
+            if (n >= 2) {
+                mt[0] = mt[0] + tau * t;
+                mt[4] = mt[4] + tau * tau * (double)(n - 1) * t;
+            }
+            if (n >= 3) {
+                mt[1] = mt[1] + tau * tau * (double)(n - 2) * t;
+                mt[2] = mt[2] - tau * tau * t;
+            }
+        }
+    } else {
+        double ex = exp(-x);
+        mt[3] = tau * (1.0 - ex);
+        mt[4] = -tau * dt * ex + tau * mt[3];
+        mt[0] = dt - mt[3];
+        mt[1] = -tau * mt[0] + mt[4];
+        mt[2] = 0.5 * dt * dt - tau * mt[0];
+    }
+}
+
+/* MicroSlope, src/mod_S2MultiscaleWall.f90:240-252 */
+static void micro_slope(int ck, const double prim[4], const double sw[4], double a[4]) {
+    a[3] = 4.0 * prim[3] * prim[3] / ((ck + 2) * prim[0]) *
+           (2.0 * sw[3] - 2.0 * prim[1] * sw[1] - 2.0 * prim[2] * sw[2] +
+            sw[0] * (prim[1] * prim[1] + prim[2] * prim[2] - 0.5 * (ck + 2) / prim[3]));
+    a[2] = 2.0 * prim[3] / prim[0] * (sw[2] - prim[2] * sw[0]) - prim[2] * a[3];
+    a[1] = 2.0 * prim[3] / prim[0] * (sw[1] - prim[1] * sw[0]) - prim[1] * a[3];
+    a[0] = sw[0] / prim[0] - prim[1] * a[1] - prim[2] * a[2] -
+           0.5 * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (ck + 2) / prim[3]) * a[3];
+}
+
+/* CalcMoment, src/mod_S2MultiscaleWall.f90:261-292 */
+static void calc_moment(int ck, const double prim[4], double mu[7], double mv[6], double mxi[3]) {
+    double mul[7], mur[7];
+    mul[0] = 0.5 * erfc(-sqrt(prim[3]) * prim[1]);
+    mul[1] = prim[1] * mul[0] + 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(PI_D * prim[3]);
+    mur[0] = 0.5 * erfc(sqrt(prim[3]) * prim[1]);
+    mur[1] = prim[1] * mur[0] - 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(PI_D * prim[3]);
+    for (int i = 2; i <= 6; ++i) {
+        mul[i] = prim[1] * mul[i - 1] + 0.5 * (i - 1) * mul[i - 2] / prim[3];
+        mur[i] = prim[1] * mur[i - 1] + 0.5 * (i - 1) * mur[i - 2] / prim[3];
+    }
+    for (int i = 0; i <= 6; ++i) mu[i] = mul[i] + mur[i];
+    mv[0] = 1.0;
+    mv[1] = prim[2];
+    for (int i = 2; i <= 5; ++i) mv[i] = prim[2] * mv[i - 1] + 0.5 * (i - 1) * mv[i - 2] / prim[3];
+    mxi[0] = 1.0;
+    mxi[1] = 0.5 * ck / prim[3];
+    mxi[2] = ck * (ck + 2.0) / (4.0 * prim[3] * prim[3]);
+}
+
+static void moment_uvxi(const double mu[7], const double mv[6], const double mxi[3], int al, int be, int de,
+                        double o[4]) {
+    o[0] = mu[al] * mv[be] * mxi[de / 2];
+    o[1] = mu[al + 1] * mv[be] * mxi[de / 2];
+    o[2] = mu[al] * mv[be + 1] * mxi[de / 2];
+    o[3] = 0.5 * (mu[al + 2] * mv[be] * mxi[de / 2] + mu[al] * mv[be + 2] * mxi[de / 2] +
+                  mu[al] * mv[be] * mxi[(de + 2) / 2]);
+}
+
+/* Moment_auvxi, src/mod_S2MultiscaleWall.f90:323-336 */
+static void moment_auvxi(const double a[4], const double mu[7], const double mv[6], const double mxi[3], int al,
+                         int be, double o[4]) {
+    double t0[4], t1[4], t2[4], t3[4], t4[4], t5[4];
+    moment_uvxi(mu, mv, mxi, al, be, 0, t0);
+    moment_uvxi(mu, mv, mxi, al + 1, be, 0, t1);
+    moment_uvxi(mu, mv, mxi, al, be + 1, 0, t2);
+    moment_uvxi(mu, mv, mxi, al + 2, be, 0, t3);
+    moment_uvxi(mu, mv, mxi, al, be + 2, 0, t4);
+    moment_uvxi(mu, mv, mxi, al, be, 2, t5);
+    for (int c = 0; c < 4; ++c)
+        o[c] = a[0] * t0[c] + a[1] * t1[c] + a[2] * t2[c] + 0.5 * a[3] * t3[c] + 0.5 * a[3] * t4[c] +
+               0.5 * a[3] * t5[c];
+}
+
+/* s2w2_wall_node (S2PhysicalGas=1), src/mod_S2MultiscaleWall.f90:45-184.
+   side=-1 bottom wall, +1 top wall; hin.. are the wall cell's reconstructed
+   face data on that wall (limited, scaled); wcell is the cell average W. */
+static int wall_node(const or_params *P, node_ws *ws, int side, double dt, double dn, const double wcell[4],
+                     const double *hin, const double *bin, const double *hn, const double *bn,
+                     const double *ht, const double *bt, const double *u, const double *v, const double *wt,
+                     double ux, double theta, double fw[4]) {
+    int n = ws->n, ck = P->ck;
+    double p = (2.0 / (double)(ck + 2)) * (wcell[3] - 0.5 * (wcell[1] * wcell[1] + wcell[2] * wcell[2]) / wcell[0]);
+    if (!(p > 0.0) || !isfinite(p)) return OR_NONFINITE;
+    double lambda = 0.5 / theta;
+    double prim_g[4] = {p / theta, 0.0, -ux, lambda};
+    double wc[4] = {wcell[0], wcell[2], -wcell[1], wcell[3]};
+    double wg[4] = {prim_g[0], 0.0, -prim_g[0] * ux,
+                    prim_g[0] * ((double)(ck + 2) / (4.0 * lambda) + 0.5 * ux * ux)};
+    double sw[4], an[4], at[4], att[4], swt[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; ++a) sw[a] = (wg[a] - wc[a]) / ((double)side * 0.5 * dn);
+    micro_slope(ck, prim_g, sw, an);
+    for (int k = 0; k < n; ++k) {
+        double vn = v[k], vt = -u[k], v2 = vn * vn + vt * vt;
+        swt[0] += wt[k] * (-ht[k]);
+        swt[1] += wt[k] * (-vn * ht[k]);
+        swt[2] += wt[k] * (-vt * ht[k]);
+        swt[3] += wt[k] * (-0.5 * (v2 * ht[k] + bt[k]));
+    }
+    /* discrete wall equilibrium with the wall state's density (wall_equilibrium, :231-238) */
+    double ww[4] = {prim_g[0], prim_g[0] * ux, 0.0,
+                    prim_g[0] * ((double)(ck + 2) * theta / 2.0 + ux * ux / 2.0)};
+    if (!solve_q_first(&ws->cl, ww, ck, u, v, wt)) return OR_NEWTON;
+    const double *mh = ws->cl.qh, *mb = ws->cl.qb;
+    micro_slope(ck, prim_g, swt, at);
+    double mu[7], mv[6], mxi[3], m1[4], m2[4];
+    calc_moment(ck, prim_g, mu, mv, mxi);
+    moment_auvxi(an, mu, mv, mxi, 1, 0, m1);
+    moment_auvxi(at, mu, mv, mxi, 0, 1, m2);
+    for (int a = 0; a < 4; ++a) sw[a] = -prim_g[0] * (m1[a] + m2[a]);
+    micro_slope(ck, prim_g, sw, att);
+    double wtau[4] = {wg[0], -wg[2], wg[1], wg[3]};
+    double tau_wall = s2_tau(P, wtau);
+    double mt[5];
+    wall_time_coefficients(dt, tau_wall, mt);
+    double *corrh = ws->s0h, *corrb = ws->s0b, *ginh = ws->mnlh, *ginb = ws->mnlb;
+    for (int k = 0; k < n; ++k) { corrh[k] = 0.0; corrb[k] = 0.0; }
+    if (P->pr != 1.0) {
+        double hl0 = 0.0, hl1 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            double vn = v[k], vt = -u[k] + ux, v2 = vn * vn + vt * vt;
+            hl0 += 0.5 * wt[k] * vn * (v2 * hin[k] + bin[k]);
+            hl1 += 0.5 * wt[k] * vt * (v2 * hin[k] + bin[k]);
+        }
+        double coeff = 0.8 * (1.0 - P->pr) * lambda * lambda / prim_g[0];
+        for (int k = 0; k < n; ++k) {
+            double vn = v[k], vt = -u[k] + ux, v2 = vn * vn + vt * vt;
+            double qdot = vn * hl0 + vt * hl1, shapeh = 2.0 * lambda * v2 - 4.0;
+            corrh[k] = coeff * qdot * shapeh * mh[k];
+            corrb[k] = coeff * qdot * (shapeh + 2.0 / (double)ck) * mb[k];
+        }
+    }
+    double incidence = 0.0, reflection = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double vn = v[k], vt = -u[k], v2 = vn * vn + vt * vt;
+        double wall = mh[k] / prim_g[0];
+        double Han = (an[0] + an[1] * vn + an[2] * vt) * mh[k] + 0.5 * an[3] * (v2 * mh[k] + mb[k]);
+        double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh[k] + 0.5 * at[3] * (v2 * mh[k] + mb[k]);
+        double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh[k] + 0.5 * att[3] * (v2 * mh[k] + mb[k]);
+        double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb[k] + 0.5 * an[3] * (v2 * mb[k] + mxi[2] * mh[k]);
+        double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb[k] + 0.5 * at[3] * (v2 * mb[k] + mxi[2] * mh[k]);
+        double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb[k] + 0.5 * att[3] * (v2 * mb[k] + mxi[2] * mh[k]);
+        ginh[k] = mt[0] * mh[k] + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin[k] -
+                  mt[4] * (vn * hn[k] + u[k] * ht[k]);
+        ginb[k] = mt[0] * mb[k] + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin[k] -
+                  mt[4] * (vn * bn[k] + u[k] * bt[k]);
+        ginh[k] = ginh[k] + mt[0] * corrh[k];
+        ginb[k] = ginb[k] + mt[0] * corrb[k];
+        if ((double)side * vn > 0.0)
+            incidence += wt[k] * vn * ginh[k];
+        else
+            reflection += dt * wt[k] * vn * wall;
+    }
+    if (!isfinite(reflection) || fabs(reflection) <= DBL_MIN) return OR_GATE;
+    double rhow = -incidence / reflection;
+    if (!isfinite(rhow) || rhow <= 0.0) return OR_GATE;
+    fw[0] = fw[1] = fw[2] = fw[3] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double fh, fb;
+        if ((double)side * v[k] > 0.0) {
+            fh = v[k] * ginh[k];
+            fb = v[k] * ginb[k];
+        } else {
+            fh = dt * v[k] * rhow * (mh[k] / prim_g[0]);
+            fb = dt * v[k] * rhow * (mb[k] / prim_g[0]);
+        }
+        ws->fh[k] = fh;
+        ws->fb[k] = fb;
+        fw[0] += wt[k] * fh;
+        fw[1] += wt[k] * u[k] * fh;
+        fw[2] += wt[k] * v[k] * fh;
+        fw[3] += wt[k] * 0.5 * ((u[k] * u[k] + v[k] * v[k]) * fh + fb);
+    }
+    double sc = fabs(incidence) > 1.0 ? fabs(incidence) : 1.0;
+    if (!(fabs(fw[0]) <= GATE * sc)) return OR_GATE;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* lifecycle                                                                   */
+/* ------------------------------------------------------------------------ */
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+or_solver *or_create(const or_params *p, const double *u, const double *v, const double *w) {
+    init_maps();
+    or_solver *s = (or_solver *)calloc(1, sizeof(or_solver));
+    s->p = *p;
+    s->nvel = p->nu * p->nv;
+    s->ncell = p->nx * p->ny;
+    int n = s->nvel, nc = s->ncell, nx = p->nx, ny = p->ny;
+    s->u = dalloc(n); s->v = dalloc(n); s->w = dalloc(n);
+    memcpy(s->u, u, sizeof(double) * n);
+    memcpy(s->v, v, sizeof(double) * n);
+    memcpy(s->w, w, sizeof(double) * n);
+    s->umax = -1e300; s->vmax = -1e300;
+    for (int k = 0; k < n; ++k) { if (u[k] > s->umax) s->umax = u[k]; if (v[k] > s->vmax) s->vmax = v[k]; }
+    s->gamma = (double)(p->ck + 4) / (double)(p->ck + 2);
+    s->W = dalloc((size_t)nc * 4);
+    s->h = dalloc((size_t)nc * n); s->b = dalloc((size_t)nc * n);
+    s->fxw = dalloc((size_t)nc * 4); s->fyw = dalloc((size_t)(ny + 1) * nx * 4);
+    s->fxh = dalloc((size_t)nc * n); s->fxb = dalloc((size_t)nc * n);
+    s->fyh = dalloc((size_t)(ny + 1) * nx * n); s->fyb = dalloc((size_t)(ny + 1) * nx * n);
+    s->mh = dalloc((size_t)nc * n); s->mb = dalloc((size_t)nc * n);
+    s->kxh = dalloc((size_t)nc * n); s->kxb = dalloc((size_t)nc * n);
+    s->kyh = dalloc((size_t)nc * n); s->kyb = dalloc((size_t)nc * n);
+    s->rate_w = dalloc((size_t)nc * 4);
+    s->stream_h = dalloc((size_t)nc * n); s->stream_b = dalloc((size_t)nc * n);
+    s->coll_h = dalloc((size_t)nc * n); s->coll_b = dalloc((size_t)nc * n);
+    s->peq_h = dalloc((size_t)nc * n); s->peq_b = dalloc((size_t)nc * n);
+    s->dW = dalloc((size_t)ny * (nx + 2) * 4);
+    s->dh = dalloc((size_t)ny * (nx + 2) * n); s->db = dalloc((size_t)ny * (nx + 2) * n);
+    s->sdW = dalloc((size_t)ny * (nx + 2) * 4);
+    s->sdh = dalloc((size_t)ny * (nx + 2) * n); s->sdb = dalloc((size_t)ny * (nx + 2) * n);
+    s->w_tilde = dalloc((size_t)nc * 4); s->tau_tilde = dalloc(nc); s->q_frozen = dalloc((size_t)nc * 2);
+    s->eq_h = dalloc((size_t)nc * n); s->eq_b = dalloc((size_t)nc * n);
+    s->low_h = dalloc((size_t)nc * n); s->low_b = dalloc((size_t)nc * n);
+    return s;
+}
+
+void or_destroy(or_solver *s) {
+    if (!s) return;
+    double *arr[] = {s->u, s->v, s->w, s->W, s->h, s->b, s->fxw, s->fyw, s->fxh, s->fxb, s->fyh, s->fyb,
+                     s->mh, s->mb, s->kxh, s->kxb, s->kyh, s->kyb, s->rate_w, s->stream_h, s->stream_b,
+                     s->coll_h, s->coll_b, s->peq_h, s->peq_b, s->dW, s->dh, s->db, s->sdW, s->sdh, s->sdb, s->w_tilde,
+                     s->tau_tilde, s->q_frozen, s->eq_h, s->eq_b, s->low_h, s->low_b};
+    for (size_t a = 0; a < sizeof arr / sizeof arr[0]; ++a) free(arr[a]);
+    free(s);
+}
+
+int or_nthreads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* Cold start: W from the rest Maxwellian (GetConserved of Prim_inf,
+   src/mod_Tools.f90:194-207) and h,b = its discrete conservative equilibrium
+   (make_mke, src/mod_S2NativeAdapter.f90:294-299). */
+int or_cold_start(or_solver *s, double rho, double ux, double tnative) {
+    int n = s->nvel, ck = s->p.ck;
+    double lam = 1.0 / tnative;
+    double w[4] = {rho, rho * ux, 0.0, 0.5 * rho / (lam * (s->gamma - 1.0)) + 0.5 * rho * (ux * ux)};
+    closure_ws c;
+    closure_ws_init(&c, n);
+    double *tmp = dalloc((size_t)8 * n);
+    int ok = solve_q_first(&c, w, ck, s->u, s->v, s->w);
+    if (ok) {
+        double res, kap[4];
+        ok = make_mke_prepared(&c, w, ck, s->p.pr, s->u, s->v, s->w, tmp, tmp + n, tmp + 2 * n, tmp + 3 * n,
+                               tmp + 4 * n, tmp + 5 * n, NULL, NULL, kap, &res, tmp + 6 * n);
+    }
+    if (ok) {
+        for (int c0 = 0; c0 < s->ncell; ++c0) {
+            memcpy(s->W + 4 * c0, w, sizeof w);
+            memcpy(s->h + (size_t)c0 * n, tmp, sizeof(double) * n);
+            memcpy(s->b + (size_t)c0 * n, tmp + n, sizeof(double) * n);
+        }
+    }
+    free(tmp);
+    closure_ws_free(&c);
+    s->have_M0 = 0;
+    s->ref_init = 0;
+    return ok ? OR_OK : OR_NEWTON;
+}
+
+int or_set_state(or_solver *s, const double *W, const double *h, const double *b) {
+    memcpy(s->W, W, sizeof(double) * 4 * s->ncell);
+    memcpy(s->h, h, sizeof(double) * s->ncell * s->nvel);
+    memcpy(s->b, b, sizeof(double) * s->ncell * s->nvel);
+    s->have_M0 = 0;
+    return OR_OK;
+}
+
+void or_get_state(const or_solver *s, double *W, double *h, double *b) {
+    if (W) memcpy(W, s->W, sizeof(double) * 4 * s->ncell);
+    if (h) memcpy(h, s->h, sizeof(double) * s->ncell * s->nvel);
+    if (b) memcpy(b, s->b, sizeof(double) * s->ncell * s->nvel);
+}
+
+/* ------------------------------------------------------------------------ */
+/* TimeStep (global varying), src/mod_Solve.f90:373-421                        */
+/* ------------------------------------------------------------------------ */
+static double time_step(const or_solver *s) {
+    const or_params *P = &s->p;
+    double tmin = 1e300;
+    for (int c = 0; c < s->ncell; ++c) {
+        double prim[4];
+        get_primary(s->gamma, s->W + 4 * c, prim);
+        double sos = sqrt(0.5 * s->gamma / prim[3]);
+        double a = fabs(prim[1]) + sos, b = fabs(prim[2]) + sos;
+        double pu = s->umax > a ? s->umax : a;
+        double pv = s->vmax > b ? s->vmax : b;
+        double area = P->dx * P->dy;
+        double dtl = P->cfl * area / (P->dy * pu + P->dx * pv);
+        if (dtl < tmin) tmin = dtl;
+    }
+    return tmin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_flux: reconstruction caches, interior faces, walls                  */
+/* (src/mod_S2D1.f90:632-959 via CartesianEvolution,                           */
+/*  src/mod_S2CartesianMPI.f90:182-234)                                        */
+/* ------------------------------------------------------------------------ */
+static int compute_flux(or_solver *s) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell;
+    size_t per = (size_t)nc * 12 * n;   /* [cell][side*3+node][k] */
+    double *ch = dalloc(per), *cb = dalloc(per), *chn = dalloc(per), *cht = dalloc(per), *cbn = dalloc(per),
+           *cbt = dalloc(per);
+    int status = OR_OK;
+    /* reconstruction, one velocity plane at a time */
+#pragma omp parallel
+    {
+        double *Fh = dalloc(nc), *Fb = dalloc(nc);
+#pragma omp for schedule(static)
+        for (int k = 0; k < n; ++k) {
+            for (int c = 0; c < nc; ++c) { Fh[c] = s->h[(size_t)c * n + k]; Fb[c] = s->b[(size_t)c * n + k]; }
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double hf[4][3], hn[4][3], ht[4][3], hv[3][3], bf[4][3], bn[4][3], bt[4][3], bv[3][3];
+                    cell_faces9(Fh, nx, ny, i, j, hf, hn, ht, hv);
+                    cell_faces9(Fb, nx, ny, i, j, bf, bn, bt, bv);
+                    int c = j * nx + i;
+                    limit_cell(Fh[c], Fb[c], P->dx, P->dy, hf, bf, hn, ht, bn, bt, hv, bv);
+                    for (int sd = 0; sd < 4; ++sd)
+                        for (int a = 0; a < 3; ++a) {
+                            size_t o = ((size_t)c * 12 + sd * 3 + a) * n + k;
+                            ch[o] = hf[sd][a]; cb[o] = bf[sd][a];
+                            chn[o] = hn[sd][a]; cht[o] = ht[sd][a];
+                            cbn[o] = bn[sd][a]; cbt[o] = bt[sd][a];
+                        }
+                }
+        }
+        free(Fh);
+        free(Fb);
+    }
+    /* macro (W) reconstruction for the face slopes, unlimited */
+    double *wf = dalloc((size_t)nc * 4 * 12), *wn = dalloc((size_t)nc * 4 * 12), *wtg = dalloc((size_t)nc * 4 * 12);
+    {
+        double *Fm = dalloc(nc);
+        for (int m = 0; m < 4; ++m) {
+            for (int c = 0; c < nc; ++c) Fm[c] = s->W[4 * c + m];
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double f[4][3], dn[4][3], dt[4][3], vol[3][3];
+                    cell_faces9(Fm, nx, ny, i, j, f, dn, dt, vol);
+                    int c = j * nx + i;
+                    for (int sd = 0; sd < 4; ++sd)
+                        for (int a = 0; a < 3; ++a) {
+                            size_t o = ((size_t)c * 4 + m) * 12 + sd * 3 + a;
+                            wf[o] = f[sd][a]; wn[o] = dn[sd][a]; wtg[o] = dt[sd][a];
+                        }
+                }
+        }
+        free(Fm);
+    }
+    double dt = s->dt;
+    const double *u = s->u, *v = s->v, *w = s->w;
+    /* interior faces: x faces for every cell, y faces between rows */
+    int nfaces = nc + nx * (ny - 1) + 2 * nx;
+#pragma omp parallel
+    {
+        node_ws ws;
+        node_ws_init(&ws, n);
+#pragma omp for schedule(dynamic, 1)
+        for (int f = 0; f < nfaces; ++f) {
+            if (status != OR_OK) continue;
+            double fwn[4], acc_w[4] = {0, 0, 0, 0};
+            double *acc_h, *acc_b;
+            if (f < nc) {
+                int j = f / nx, i = f % nx, cl = j * nx + i, cr = j * nx + ip(i + 1, nx);
+                acc_h = s->fxh + (size_t)cl * n;
+                acc_b = s->fxb + (size_t)cl * n;
+                for (int k = 0; k < n; ++k) { acc_h[k] = 0.0; acc_b[k] = 0.0; }
+                for (int a = 0; a < 3; ++a) {
+                    double mnl[4], mnr[4], mtg[4];
+                    for (int m = 0; m < 4; ++m) {
+                        size_t ol = ((size_t)cl * 4 + m) * 12 + 1 * 3 + a, orr = ((size_t)cr * 4 + m) * 12 + 0 * 3 + a;
+                        mnl[m] = wn[ol] / P->dx;
+                        mnr[m] = wn[orr] / P->dx;
+                        mtg[m] = 0.5 * (wtg[ol] + wtg[orr]) / P->dy;
+                    }
+                    size_t L = ((size_t)cl * 12 + 1 * 3 + a) * n, R = ((size_t)cr * 12 + 0 * 3 + a) * n;
+                    int st = ugks_node(P, &ws, 1, dt, ch + L, ch + R, cb + L, cb + R, chn + L, chn + R, cht + L,
+                                       cht + R, cbn + L, cbn + R, cbt + L, cbt + R, mnl, mnr, mtg, u, v, w, fwn);
+                    if (st != OR_OK) { status = st; break; }
+                    for (int k = 0; k < n; ++k) { acc_h[k] += GW[a] * ws.fh[k]; acc_b[k] += GW[a] * ws.fb[k]; }
+                    for (int m = 0; m < 4; ++m) acc_w[m] += GW[a] * fwn[m];
+                }
+                memcpy(s->fxw + 4 * cl, acc_w, sizeof acc_w);
+            } else if (f < nc + nx * (ny - 1)) {
+                int g = f - nc, j = g / nx, i = g % nx;   /* face between row j and j+1 */
+                int cl = j * nx + i, cr = (j + 1) * nx + i;
+                size_t fo = (size_t)(j + 1) * nx + i;     /* south face of row j+1 */
+                acc_h = s->fyh + fo * n;
+                acc_b = s->fyb + fo * n;
+                for (int k = 0; k < n; ++k) { acc_h[k] = 0.0; acc_b[k] = 0.0; }
+                for (int a = 0; a < 3; ++a) {
+                    double mnl[4], mnr[4], mtg[4];
+                    for (int m = 0; m < 4; ++m) {
+                        size_t ol = ((size_t)cl * 4 + m) * 12 + 3 * 3 + a, orr = ((size_t)cr * 4 + m) * 12 + 2 * 3 + a;
+                        mnl[m] = wn[ol] / P->dy;
+                        mnr[m] = wn[orr] / P->dy;
+                        mtg[m] = 0.5 * (wtg[ol] + wtg[orr]) / P->dx;
+                    }
+                    size_t L = ((size_t)cl * 12 + 3 * 3 + a) * n, R = ((size_t)cr * 12 + 2 * 3 + a) * n;
+                    int st = ugks_node(P, &ws, 2, dt, ch + L, ch + R, cb + L, cb + R, chn + L, chn + R, cht + L,
+                                       cht + R, cbn + L, cbn + R, cbt + L, cbt + R, mnl, mnr, mtg, u, v, w, fwn);
+                    if (st != OR_OK) { status = st; break; }
+                    for (int k = 0; k < n; ++k) { acc_h[k] += GW[a] * ws.fh[k]; acc_b[k] += GW[a] * ws.fb[k]; }
+                    for (int m = 0; m < 4; ++m) acc_w[m] += GW[a] * fwn[m];
+                }
+                memcpy(s->fyw + 4 * fo, acc_w, sizeof acc_w);
+            } else {
+                int g = f - nc - nx * (ny - 1), top = g >= nx, i = g % nx;
+                int j = top ? ny - 1 : 0, sd = top ? 3 : 2, side = top ? 1 : -1;
+                int c = j * nx + i;
+                size_t fo = (size_t)(top ? ny : 0) * nx + i;
+                acc_h = s->fyh + fo * n;
+                acc_b = s->fyb + fo * n;
+                for (int k = 0; k < n; ++k) { acc_h[k] = 0.0; acc_b[k] = 0.0; }
+                for (int a = 0; a < 3; ++a) {
+                    size_t o = ((size_t)c * 12 + sd * 3 + a) * n;
+                    int st = wall_node(P, &ws, side, dt, P->dy, s->W + 4 * c, ch + o, cb + o, chn + o, cbn + o,
+                                       cht + o, cbt + o, u, v, w, P->wall_ux[top], 0.5 * P->wall_tn[top], fwn);
+                    if (st != OR_OK) { status = st; break; }
+                    for (int k = 0; k < n; ++k) { acc_h[k] += GW[a] * ws.fh[k]; acc_b[k] += GW[a] * ws.fb[k]; }
+                    for (int m = 0; m < 4; ++m) acc_w[m] += GW[a] * fwn[m];
+                }
+                memcpy(s->fyw + 4 * fo, acc_w, sizeof acc_w);
+            }
+        }
+        node_ws_free(&ws);
+    }
+    free(ch); free(cb); free(chn); free(cht); free(cbn); free(cbt);
+    free(wf); free(wn); free(wtg);
+    return status;
+}
+
+/* build_targets: 3x3 volume nodes of the WENO tensor W, mean-corrected, each
+   closed by make_mke and averaged with the Gauss weights,
+   src/mod_S2D1.f90:1246-1300 */
+static int build_targets(or_solver *s) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell;
+    double *nodes = dalloc((size_t)nc * 36);   /* [cell][m][a][c] */
+    double *Fm = dalloc(nc);
+    for (int m = 0; m < 4; ++m) {
+        for (int c = 0; c < nc; ++c) Fm[c] = s->W[4 * c + m];
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double f[4][3], dn[4][3], dt[4][3], vol[3][3];
+                cell_faces9(Fm, nx, ny, i, j, f, dn, dt, vol);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) nodes[((size_t)(j * nx + i) * 4 + m) * 9 + a * 3 + b] = vol[a][b];
+            }
+    }
+    free(Fm);
+    int status = OR_OK;
+#pragma omp parallel
+    {
+        closure_ws c;
+        closure_ws_init(&c, n);
+        double *t = dalloc((size_t)8 * n);
+#pragma omp for schedule(dynamic, 1)
+        for (int cell = 0; cell < nc; ++cell) {
+            if (status != OR_OK) continue;
+            const double *W = s->W + 4 * cell;
+            double avg[4] = {0, 0, 0, 0};
+            for (int cc = 0; cc < 3; ++cc)
+                for (int a = 0; a < 3; ++a)
+                    for (int m = 0; m < 4; ++m) avg[m] += GW[a] * GW[cc] * nodes[((size_t)cell * 4 + m) * 9 + a * 3 + cc];
+            double *mh = s->mh + (size_t)cell * n, *mb = s->mb + (size_t)cell * n, *kxh = s->kxh + (size_t)cell * n,
+                   *kxb = s->kxb + (size_t)cell * n, *kyh = s->kyh + (size_t)cell * n, *kyb = s->kyb + (size_t)cell * n;
+            for (int k = 0; k < n; ++k) mh[k] = mb[k] = kxh[k] = kxb[k] = kyh[k] = kyb[k] = 0.0;
+            for (int cc = 0; cc < 3; ++cc)
+                for (int a = 0; a < 3; ++a) {
+                    double wq[4], res, kap[4];
+                    for (int m = 0; m < 4; ++m) wq[m] = nodes[((size_t)cell * 4 + m) * 9 + a * 3 + cc] + W[m] - avg[m];
+                    double wt2 = GW[a] * GW[cc];
+                    if (!solve_q_first(&c, wq, P->ck, s->u, s->v, s->w)) { status = OR_NEWTON; break; }
+                    if (!make_mke_prepared(&c, wq, P->ck, P->pr, s->u, s->v, s->w, t, t + n, t + 2 * n, t + 3 * n,
+                                           t + 4 * n, t + 5 * n, NULL, NULL, kap, &res, t + 6 * n)) {
+                        status = OR_GATE;
+                        break;
+                    }
+                    for (int k = 0; k < n; ++k) {
+                        mh[k] += wt2 * t[k]; mb[k] += wt2 * t[n + k];
+                        kxh[k] += wt2 * t[2 * n + k]; kxb[k] += wt2 * t[3 * n + k];
+                        kyh[k] += wt2 * t[4 * n + k]; kyb[k] += wt2 * t[5 * n + k];
+                    }
+                }
+        }
+        free(t);
+        closure_ws_free(&c);
+    }
+    free(nodes);
+    return status;
+}
+
+/* ComputeResidual (AssembleMacroResidual, AssembleMicroFluxResidual,
+   AssembleSteadyRateResiduals with S2NativePhysicalCollision),
+   src/mod_Residual.f90:17-132, src/mod_S2NativeAdapter.f90:735-759.  Face
+   fluxes carry dt and are per unit length; the cell update is
+   (F_w - F_e)*dy + (F_s - F_n)*dx over the area. */
+static int compute_residual(or_solver *s) {
+    const or_params *P = &s->p;
+    int nx = P->nx, n = s->nvel;
+    double area = P->dx * P->dy, inv_dt = 1.0 / s->dt;
+    int status = OR_OK;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s->ncell; ++c) {
+        int j = c / nx, i = c % nx, cw = j * nx + ip(i - 1, nx);
+        size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
+        double ainv = 1.0 / area;
+        for (int m = 0; m < 4; ++m) {
+            double fc = (s->fxw[4 * cw + m] * P->dy - s->fxw[4 * c + m] * P->dy + s->fyw[4 * fs + m] * P->dx -
+                         s->fyw[4 * fn + m] * P->dx) / area;
+            s->rate_w[4 * c + m] = fc * inv_dt;
+        }
+        for (int k = 0; k < n; ++k) {
+            double rh = (s->fxh[(size_t)cw * n + k] * P->dy - s->fxh[(size_t)c * n + k] * P->dy +
+                         s->fyh[fs * n + k] * P->dx - s->fyh[fn * n + k] * P->dx) * ainv;
+            double rb = (s->fxb[(size_t)cw * n + k] * P->dy - s->fxb[(size_t)c * n + k] * P->dy +
+                         s->fyb[fs * n + k] * P->dx - s->fyb[fn * n + k] * P->dx) * ainv;
+            s->stream_h[(size_t)c * n + k] = rh * inv_dt;
+            s->stream_b[(size_t)c * n + k] = rb * inv_dt;
+        }
+        /* physical collision */
+        const double *W = s->W + 4 * c;
+        double q[2];
+        const double *h = s->h + (size_t)c * n, *b = s->b + (size_t)c * n;
+        if (!s2_heat_diff(W, P->ck, n, h, s->mh + (size_t)c * n, b, s->mb + (size_t)c * n, s->u, s->v, s->w, q)) {
+            status = OR_NONFINITE;
+            continue;
+        }
+        double tau = s2_tau(P, W);
+        double mom[4] = {0, 0, 0, 0};
+        for (int k = 0; k < n; ++k) {
+            size_t o = (size_t)c * n + k;
+            double eh = s->mh[o] + s->kxh[o] * q[0] + s->kyh[o] * q[1];
+            double eb = s->mb[o] + s->kxb[o] * q[0] + s->kyb[o] * q[1];
+            s->peq_h[o] = eh; s->peq_b[o] = eb;
+            s->coll_h[o] = (eh - h[k]) / tau;
+            s->coll_b[o] = (eb - b[k]) / tau;
+            mom[0] += s->w[k] * s->coll_h[o];
+            mom[1] += s->w[k] * s->u[k] * s->coll_h[o];
+            mom[2] += s->w[k] * s->v[k] * s->coll_h[o];
+            mom[3] += s->w[k] * ((s->u[k] * s->u[k] + s->v[k] * s->v[k]) * s->coll_h[o] + s->coll_b[o]);
+        }
+        mom[3] *= 0.5;
+        double mx = 0.0, wm = 1.0;
+        for (int m = 0; m < 4; ++m) { if (fabs(mom[m]) > mx) mx = fabs(mom[m]); if (fabs(W[m]) > wm) wm = fabs(W[m]); }
+        if (tau * mx / wm > GATE) status = OR_GATE;
+        /* stream moments vs rate_w (src/mod_Residual.f90:104-109) */
+        double sm[4];
+        moment_2d(n, s->stream_h + (size_t)c * n, s->stream_b + (size_t)c * n, s->u, s->v, s->w, sm);
+        double rm = 1.0, e = 0.0;
+        for (int m = 0; m < 4; ++m) {
+            if (fabs(s->rate_w[4 * c + m]) > rm) rm = fabs(s->rate_w[4 * c + m]);
+            if (fabs(sm[m] - s->rate_w[4 * c + m]) > e) e = fabs(sm[m] - s->rate_w[4 * c + m]);
+        }
+        if (e / rm > GATE) status = OR_GATE;
+    }
+    return status;
+}
+
+/* UpdateImplicitResidualDiagnostics (UnifiedPhysicalOperators branch),
+   src/mod_ImplicitDiagnostics.f90:223-322 */
+static void update_norms(or_solver *s) {
+    const or_params *P = &s->p;
+    int n = s->nvel;
+    double area = P->dx * P->dy;
+    double num[4] = {0, 0, 0, 0}, den[4] = {0, 0, 0, 0}, wsum = 0.0, ms[4] = {0, 0, 0, 0}, mass = 0.0;
+    for (int c = 0; c < s->ncell; ++c) {
+        for (int m = 0; m < 4; ++m) {
+            double t = area * s->rate_w[4 * c + m];
+            num[m] += t * t;
+            den[m] += area * fabs(s->W[4 * c + m] + s->dt * s->rate_w[4 * c + m]);
+        }
+        wsum += area * area;
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (int k = 0; k < n; ++k) {
+            size_t o = (size_t)c * n + k;
+            double dh = s->stream_h[o] + s->coll_h[o], db = s->stream_b[o] + s->coll_b[o];
+            s0 += area * s->w[k] * dh * dh;
+            s1 += area * s->w[k] * db * db;
+        }
+        for (int k = 0; k < n; ++k) {
+            size_t o = (size_t)c * n + k;
+            double a2 = s->h[o] * s->h[o], e2 = s->peq_h[o] * s->peq_h[o];
+            s2 += s->w[k] * (a2 > e2 ? a2 : e2);
+            a2 = s->b[o] * s->b[o]; e2 = s->peq_b[o] * s->peq_b[o];
+            s3 += s->w[k] * (a2 > e2 ? a2 : e2);
+        }
+        ms[0] += s0; ms[1] += s1; ms[2] += area * s2; ms[3] += area * s3;
+        mass += s->rate_w[4 * c] * area;
+    }
+    double *o = s->norms;
+    double N = (double)(P->nx * P->ny), ordinary = den[0] > den[3] ? den[0] : den[3];
+    if (ordinary < SMV) ordinary = SMV;
+    for (int m = 0; m < 4; ++m) {
+        o[4 + m] = sqrt(N * num[m]) / (den[m] + SMV);
+        o[8 + m] = sqrt(num[m] / (wsum > SMV ? wsum : SMV));
+        o[m] = (den[m] <= sqrt(DBL_EPSILON) * ordinary) ? o[8 + m] : o[4 + m];
+    }
+    double h_raw = sqrt(ms[0] / (ms[2] > SMV ? ms[2] : SMV));
+    double b_raw = sqrt(ms[1] / (ms[3] > SMV ? ms[3] : SMV));
+    if (!s->ref_init) { s->reference_h = h_raw; s->reference_b = b_raw; s->ref_init = 1; }
+    double nz = 100.0 * P->residual_tol;
+    if (nz < 1000.0 * DBL_EPSILON) nz = 1000.0 * DBL_EPSILON;
+    o[14] = s->reference_h > DBL_MIN ? h_raw / s->reference_h : h_raw;
+    o[15] = s->reference_b > DBL_MIN ? b_raw / s->reference_b : b_raw;
+    o[16] = h_raw; o[17] = b_raw;
+    o[12] = s->reference_h <= nz ? h_raw : h_raw / s->reference_h;
+    o[13] = s->reference_b <= nz ? b_raw : b_raw / s->reference_b;
+    o[19] = fabs(mass);
+    o[20] = sqrt(ms[0]); o[21] = sqrt(ms[1]);
+    o[22] = s->reference_h; o[23] = s->reference_b;
+}
+
+int or_operator(or_solver *s) {
+    s->dt = time_step(s);
+    int st = build_targets(s);
+    if (st != OR_OK) return st;
+    st = compute_flux(s);
+    if (st != OR_OK) return st;
+    st = compute_residual(s);
+    if (st != OR_OK) return st;
+    update_norms(s);
+    return OR_OK;
+}
+
+void or_get_operator(const or_solver *s, double *dt, double *rate_w, double *stream_h, double *stream_b,
+                     double *coll_h, double *coll_b, double *xflux_h, double *xflux_b, double *yflux_h,
+                     double *yflux_b, double *xflux_w, double *yflux_w) {
+    size_t cv = (size_t)s->ncell * s->nvel, fy = (size_t)(s->p.ny + 1) * s->p.nx;
+    if (dt) *dt = s->dt;
+    if (rate_w) memcpy(rate_w, s->rate_w, sizeof(double) * 4 * s->ncell);
+    if (stream_h) memcpy(stream_h, s->stream_h, sizeof(double) * cv);
+    if (stream_b) memcpy(stream_b, s->stream_b, sizeof(double) * cv);
+    if (coll_h) memcpy(coll_h, s->coll_h, sizeof(double) * cv);
+    if (coll_b) memcpy(coll_b, s->coll_b, sizeof(double) * cv);
+    if (xflux_h) memcpy(xflux_h, s->fxh, sizeof(double) * cv);
+    if (xflux_b) memcpy(xflux_b, s->fxb, sizeof(double) * cv);
+    if (yflux_h) memcpy(yflux_h, s->fyh, sizeof(double) * fy * s->nvel);
+    if (yflux_b) memcpy(yflux_b, s->fyb, sizeof(double) * fy * s->nvel);
+    if (xflux_w) memcpy(xflux_w, s->fxw, sizeof(double) * 4 * s->ncell);
+    if (yflux_w) memcpy(yflux_w, s->fyw, sizeof(double) * 4 * fy);
+}
+
+void or_get_norms(const or_solver *s, double *out24) { memcpy(out24, s->norms, sizeof(double) * 24); }
+
+/* ------------------------------------------------------------------------ */
+/* implicit update                                                             */
+/* ------------------------------------------------------------------------ */
+/* y block of row j when the grid is split into np_y equal slabs, as the
+   reference's MPI layout NP_X=1, NP_Y=np_y does (src/mod_MPI.f90:33-53) */
+static inline int yblock(const or_solver *s, int j) {
+    int nb = s->p.np_y > 1 ? s->p.np_y : 1;
+    return j / (s->p.ny / nb);
+}
+static inline double *sdWp(or_solver *s, int i, int j) { return s->sdW + ((size_t)j * (s->p.nx + 2) + i + 1) * 4; }
+static inline double *sdhp(or_solver *s, int i, int j) { return s->sdh + ((size_t)j * (s->p.nx + 2) + i + 1) * s->nvel; }
+static inline double *sdbp(or_solver *s, int i, int j) { return s->sdb + ((size_t)j * (s->p.nx + 2) + i + 1) * s->nvel; }
+static inline double *dWp(or_solver *s, int i, int j) { return s->dW + ((size_t)j * (s->p.nx + 2) + i + 1) * 4; }
+static inline double *dhp(or_solver *s, int i, int j) { return s->dh + ((size_t)j * (s->p.nx + 2) + i + 1) * s->nvel; }
+static inline double *dbp(or_solver *s, int i, int j) { return s->db + ((size_t)j * (s->p.nx + 2) + i + 1) * s->nvel; }
+
+/* EulerFluxNormal, src/mod_ImplicitCommon.f90:22-30 */
+static void euler_flux(double gamma, const double w[4], double nx, double ny, double f[4]) {
+    double u = w[1] / w[0], v = w[2] / w[0], un = u * nx + v * ny;
+    double pr = (gamma - 1.0) * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+    f[0] = w[0] * un;
+    f[1] = w[1] * un + pr * nx;
+    f[2] = w[2] * un + pr * ny;
+    f[3] = (w[3] + pr) * un;
+}
+
+/* ZhuMovingWallMap, src/mod_ImplicitBoundary.f90:187-194 */
+static void zhu_map(const double w[4], double uw, double vw, double g[4]) {
+    g[0] = w[0];
+    g[1] = 2.0 * uw * w[0] - w[1];
+    g[2] = 2.0 * vw * w[0] - w[2];
+    g[3] = w[3] + 2.0 * (uw * uw + vw * vw) * w[0] - 2.0 * uw * w[1] - 2.0 * vw * w[2];
+}
+
+/* face geometry of cell (i,j) face fnum (0 W, 1 E, 2 S, 3 N): outward normal,
+   wall flag and wall index */
+static void face_geom(const or_solver *s, int j, int fnum, double *nxo, double *nyo, double *len, int *wall,
+                      int *top) {
+    const or_params *P = &s->p;
+    *wall = 0; *top = 0;
+    switch (fnum) {
+    case 0: *nxo = -1.0; *nyo = 0.0; *len = P->dy; break;
+    case 1: *nxo = 1.0; *nyo = 0.0; *len = P->dy; break;
+    case 2: *nxo = 0.0; *nyo = -1.0; *len = P->dx; *wall = (j == 0); break;
+    default: *nxo = 0.0; *nyo = 1.0; *len = P->dx; *wall = (j == P->ny - 1); *top = 1; break;
+    }
+}
+
+/* ImplicitSpectralRadius, src/mod_ImplicitMacro.f90:15-55 */
+static double spectral_radius(const or_solver *s, const double wi[4], const double wj[4], double nxo, double nyo,
+                              double dist) {
+    double pi_[4], pj[4];
+    get_primary(s->gamma, wi, pi_);
+    get_primary(s->gamma, wj, pj);
+    double rho = 0.5 * (pi_[0] + pj[0]), ua = 0.5 * (pi_[1] + pj[1]), va = 0.5 * (pi_[2] + pj[2]);
+    double p_i = 0.5 * pi_[0] / pi_[3], p_j = 0.5 * pj[0] / pj[3], pa = 0.5 * (p_i + p_j);
+    double ti = get_tau(&s->p, pi_), tj = get_tau(&s->p, pj);
+    double mu = 0.5 * (ti * p_i + tj * p_j);
+    double c = sqrt(s->gamma * pa / rho);
+    double d = dist > 1000.0 * DBL_EPSILON ? dist : 1000.0 * DBL_EPSILON;
+    return fabs(ua * nxo + va * nyo) + c + 2.0 * mu / (rho * d);
+}
+
+/* RelaxMacroPoint with GetMacroFaceTerms, src/mod_ImplicitMacro.f90:90-123,187-221 */
+static int relax_macro(or_solver *s, int i, int j) {
+    const or_params *P = &s->p;
+    int nx = P->nx;
+    const double *wi = s->W + 4 * cidx(s, i, j);
+    double *di = dWp(s, i, j);
+    double sdt[4] = {0, 0, 0, 0}, sgn[4] = {0, 0, 0, 0}, sg = 0.0;
+    for (int fnum = 0; fnum < 4; ++fnum) {
+        double nxo, nyo, len, wj[4], dnb[4], dist;
+        int wall, top;
+        face_geom(s, j, fnum, &nxo, &nyo, &len, &wall, &top);
+        if (wall) {
+            double uw = P->wall_ux[top];
+            zhu_map(wi, uw, 0.0, wj);
+            zhu_map(di, uw, 0.0, dnb);
+            dist = P->dy;
+        } else {
+            int ni = i, nj = j;
+            if (fnum == 0) ni = i - 1;
+            else if (fnum == 1) ni = i + 1;
+            else if (fnum == 2) nj = j - 1;
+            else nj = j + 1;
+            /* periodic x: W of the wrapped cell; delta from the halo column */
+            memcpy(wj, s->W + 4 * cidx(s, ip(ni, nx), nj), sizeof wj);
+            if (yblock(s, nj) != yblock(s, j))
+                memcpy(dnb, sdWp(s, ni, nj), sizeof dnb);   /* neighbour rank's halo */
+            else
+                memcpy(dnb, dWp(s, ni, nj), sizeof dnb);
+            dist = (fnum < 2) ? P->dx : P->dy;
+        }
+        double a[4], b[4], fnew[4], fold[4], ti[4], tj[4];
+        for (int m = 0; m < 4; ++m) a[m] = wi[m] + di[m];
+        euler_flux(s->gamma, a, nxo, nyo, fnew);
+        euler_flux(s->gamma, wi, nxo, nyo, fold);
+        for (int m = 0; m < 4; ++m) ti[m] = fnew[m] - fold[m];
+        for (int m = 0; m < 4; ++m) b[m] = wj[m] + dnb[m];
+        euler_flux(s->gamma, b, nxo, nyo, fnew);
+        euler_flux(s->gamma, wj, nxo, nyo, fold);
+        for (int m = 0; m < 4; ++m) tj[m] = fnew[m] - fold[m];
+        double g = spectral_radius(s, wi, wj, nxo, nyo, dist);
+        for (int m = 0; m < 4; ++m) {
+            sdt[m] += len * (ti[m] + tj[m]);
+            sgn[m] += len * g * dnb[m];
+        }
+        sg += len * g;
+    }
+    double area = P->dx * P->dy;
+    double dtp = s->dt * P->ncfl / P->cfl;
+    double diag = 1.0 / dtp + 0.5 * sg / area;
+    double rhs[4];
+    for (int m = 0; m < 4; ++m) rhs[m] = s->rate_w[4 * cidx(s, i, j) + m] - 0.5 * sdt[m] / area + 0.5 * sgn[m] / area;
+    if (!(isfinite(diag) && diag > 0.0)) return 0;
+    for (int m = 0; m < 4; ++m) { if (!isfinite(rhs[m])) return 0; }
+    for (int m = 0; m < 4; ++m) di[m] = rhs[m] / diag;
+    return 1;
+}
+
+/* halo exchange of a delta field (MPIExchangeField3D): x-periodic halo
+   columns, plus a snapshot that neighbouring y blocks read until the next
+   exchange */
+static void exchange_x(double *f, double *snap, int nx, int ny, int ncomp) {
+    for (int j = 0; j < ny; ++j) {
+        double *row = f + (size_t)j * (nx + 2) * ncomp;
+        memcpy(row, row + (size_t)nx * ncomp, sizeof(double) * ncomp);
+        memcpy(row + (size_t)(nx + 1) * ncomp, row + (size_t)1 * ncomp, sizeof(double) * ncomp);
+    }
+    memcpy(snap, f, sizeof(double) * (size_t)ny * (nx + 2) * ncomp);
+}
+
+/* SolveMacroPR2, src/mod_ImplicitMacro.f90:57-88 */
+static int solve_macro_pr2(or_solver *s) {
+    int nx = s->p.nx, ny = s->p.ny;
+    exchange_x(s->dW, s->sdW, nx, ny, 4);
+    for (int sweep = 0; sweep < s->p.macro_sweeps; ++sweep) {
+        int ok = 1;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) ok &= relax_macro(s, i, j);
+        for (int j = ny - 1; j >= 0; --j)
+            for (int i = nx - 1; i >= 0; --i) ok &= relax_macro(s, i, j);
+        if (!ok) return 0;
+        exchange_x(s->dW, s->sdW, nx, ny, 4);
+    }
+    return 1;
+}
+
+/* DiscreteMaxwell_Vec + ShakhovPart_Vec, src/mod_Tools.f90:18-36,72-107 */
+static void maxwell_shakhov(const or_solver *s, const double prim[4], const double q[2], double *eh, double *eb) {
+    int ck = s->p.ck, n = s->nvel;
+    double pre = prim[0] * (prim[3] / PI_D), bf = ck / (2.0 * prim[3]);
+    double coeff = 0.8 * (1.0 - s->p.pr) * prim[3] * prim[3] / prim[0];
+    for (int k = 0; k < n; ++k) {
+        double dvn = s->u[k] - prim[1], dvt = s->v[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
+        double hk = pre * exp(-prim[3] * v2);
+        double bk = hk * bf;
+        double qdot = dvn * q[0] + dvt * q[1];
+        double shH = 2.0 * prim[3] * v2 - 4.0, shB = shH + 2.0 / ck;
+        eh[k] = hk + coeff * qdot * shH * hk;
+        eb[k] = bk + coeff * qdot * shB * bk;
+    }
+}
+
+/* GetHeatFlux_Vec, src/mod_Tools.f90:355-373 */
+static void heat_flux_vec(const or_solver *s, const double *h, const double *b, const double prim[4], double q[2]) {
+    q[0] = q[1] = 0.0;
+    for (int k = 0; k < s->nvel; ++k) {
+        double dvn = s->u[k] - prim[1], dvt = s->v[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
+        q[0] += 0.5 * s->w[k] * dvn * (v2 * h[k] + b[k]);
+        q[1] += 0.5 * s->w[k] * dvt * (v2 * h[k] + b[k]);
+    }
+}
+
+/* ProjectToConservedMoments with reference = the unprojected values,
+   src/mod_ImplicitCommon.f90:42-82 */
+static int project_moments(const or_solver *s, double *h, double *b, const double target[4]) {
+    int n = s->nvel;
+    double mom[4], rhs[4], gram[16] = {0}, lam[4];
+    double sh = 1.0, sb = 1.0;
+    for (int k = 0; k < n; ++k) { if (fabs(h[k]) > sh) sh = fabs(h[k]); if (fabs(b[k]) > sb) sb = fabs(b[k]); }
+    moment_2d(n, h, b, s->u, s->v, s->w, mom);
+    for (int a = 0; a < 4; ++a) rhs[a] = target[a] - mom[a];
+    double *wh = (double *)malloc(sizeof(double) * 2 * n), *wb = wh + n;
+    for (int k = 0; k < n; ++k) {
+        double kh[4] = {1.0, s->u[k], s->v[k], 0.5 * (s->u[k] * s->u[k] + s->v[k] * s->v[k])};
+        double kb[4] = {0.0, 0.0, 0.0, 0.5};
+        wh[k] = fabs(h[k]) > DBL_EPSILON * sh ? fabs(h[k]) : DBL_EPSILON * sh;
+        wb[k] = fabs(b[k]) > DBL_EPSILON * sb ? fabs(b[k]) : DBL_EPSILON * sb;
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 4; ++c) gram[c * 4 + a] += s->w[k] * (wh[k] * kh[a] * kh[c] + wb[k] * kb[a] * kb[c]);
+    }
+    if (!solve_dense4_gj(gram, rhs, lam)) { free(wh); return 0; }
+    for (int k = 0; k < n; ++k) {
+        double kh[4] = {1.0, s->u[k], s->v[k], 0.5 * (s->u[k] * s->u[k] + s->v[k] * s->v[k])};
+        h[k] = h[k] + wh[k] * (kh[0] * lam[0] + kh[1] * lam[1] + kh[2] * lam[2] + kh[3] * lam[3]);
+        b[k] = b[k] + 0.5 * wb[k] * lam[3];
+    }
+    free(wh);
+    moment_2d(n, h, b, s->u, s->v, s->w, mom);
+    double rs = 1.0, e = 0.0;
+    for (int a = 0; a < 4; ++a) { if (fabs(target[a]) > rs) rs = fabs(target[a]); if (fabs(mom[a] - target[a]) > e) e = fabs(mom[a] - target[a]); }
+    return e <= 1.0e-10 * rs;
+}
+
+/* BuildPredictedEquilibrium (HighOrderImplicit branch), src/mod_ImplicitMicro.f90:43-88 */
+static int build_predicted_equilibrium(or_solver *s) {
+    int n = s->nvel, ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (int c = 0; c < s->ncell; ++c) {
+        double po[4], pt[4];
+        get_primary(s->gamma, s->W + 4 * c, po);
+        double *q = s->q_frozen + 2 * c;
+        heat_flux_vec(s, s->h + (size_t)c * n, s->b + (size_t)c * n, po, q);
+        get_primary(s->gamma, s->w_tilde + 4 * c, pt);
+        s->tau_tilde[c] = get_tau(&s->p, pt);
+        double *eh = s->eq_h + (size_t)c * n, *eb = s->eq_b + (size_t)c * n;
+        maxwell_shakhov(s, pt, q, eh, eb);
+        int pok = project_moments(s, eh, eb, s->w_tilde + 4 * c);
+        double *lh = s->low_h + (size_t)c * n, *lb = s->low_b + (size_t)c * n;
+        maxwell_shakhov(s, po, q, lh, lb);
+        int ook = project_moments(s, lh, lb, s->W + 4 * c);
+        double to = get_tau(&s->p, po);
+        for (int k = 0; k < n; ++k) {
+            lh[k] = (lh[k] - s->h[(size_t)c * n + k]) / to;
+            lb[k] = (lb[k] - s->b[(size_t)c * n + k]) / to;
+        }
+        int fin = 1;
+        for (int k = 0; k < n; ++k) fin &= isfinite(eh[k]) && isfinite(eb[k]);
+        ok = ok && pok && ook && fin && s->tau_tilde[c] > 0.0;
+    }
+    return ok;
+}
+
+/* ComputeDiffuseWallFaceDelta for the frozen wall closure,
+   src/mod_ImplicitBoundary.f90:282-326.  Writes the reflected-face
+   increment (only the incoming half, selector=1, is ever read). */
+static int wall_face_delta(const or_solver *s, int top, const double *dhi, const double *dbi, double *fh,
+                           double *fb) {
+    const or_params *P = &s->p;
+    int n = s->nvel, ck = P->ck;
+    double nyo = top ? 1.0 : -1.0;            /* outward normal (0, nyo) */
+    double uw = P->wall_ux[top], lam = 1.0 / P->wall_tn[top];
+    /* LocalFrame(prim_wall, 0, nyo): normal velocity 0, tangential -uw*nyo */
+    double pl1 = 0.0, pl2 = -uw * nyo;
+    double pre = 1.0 * (lam / PI_D), bf = ck / (2.0 * lam);
+    double inc = 0.0, refl = 0.0, scale = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double vn = s->v[k] * nyo, vt = -s->u[k] * nyo;
+        double sel = vn < 0.0 ? 1.0 : 0.0;
+        double dvn = vn - pl1, dvt = vt - pl2;
+        double hu = pre * exp(-lam * (dvn * dvn + dvt * dvt));
+        fh[k] = hu;
+        fb[k] = hu * bf;
+        inc += s->w[k] * vn * dhi[k] * (1.0 - sel);
+        refl += s->w[k] * vn * hu * sel;
+        scale += fabs(s->w[k] * vn * hu * sel);
+    }
+    double smin = scale > DBL_MIN ? scale : DBL_MIN;
+    if (fabs(refl) <= 1000.0 * DBL_EPSILON * smin) return 0;
+    double drho = -inc / refl;
+    for (int k = 0; k < n; ++k) {
+        double vn = s->v[k] * nyo;
+        double sel = vn < 0.0 ? 1.0 : 0.0;
+        fh[k] = drho * fh[k] * sel + dhi[k] * (1.0 - sel);
+        fb[k] = drho * fb[k] * sel + dbi[k] * (1.0 - sel);
+    }
+    return 1;
+}
+
+/* RelaxMicroPoint for all velocities of cell (i,j) in the given order, with
+   the wall closure frozen from the cell's delta before this relaxation,
+   src/mod_ImplicitMicro.f90:366-487,647-791 */
+static int relax_micro_cell(or_solver *s, int i, int j, int backward, double *wfh, double *wfb) {
+    const or_params *P = &s->p;
+    int n = s->nvel, ny = P->ny, c = cidx(s, i, j);
+    int bottom = (j == 0), topw = (j == ny - 1);
+    double *dhc = dhp(s, i, j), *dbc = dbp(s, i, j);
+    /* frozen wall closure for faces 2 (bottom) and 3 (top) */
+    double *bh = wfh, *bb = wfb, *th = wfh + n, *tb = wfb + n;
+    if (bottom && !wall_face_delta(s, 0, dhc, dbc, bh, bb)) return 0;
+    if (topw && !wall_face_delta(s, 1, dhc, dbc, th, tb)) return 0;
+    double area = P->dx * P->dy, dtp = s->dt * P->ncfl / P->cfl, tt = s->tau_tilde[c];
+    for (int kk = 0; kk < n; ++kk) {
+        int k = backward ? n - 1 - kk : kk;
+        size_t o = (size_t)c * n + k;
+        double rh = s->stream_h[o] + (s->eq_h[o] - s->h[o]) / tt;
+        double rb = s->stream_b[o] + (s->eq_b[o] - s->b[o]) / tt;
+        rh = rh + s->coll_h[o] - s->low_h[o];
+        rb = rb + s->coll_b[o] - s->low_b[o];
+        double diag = 1.0 / dtp + 1.0 / tt;
+        for (int fnum = 0; fnum < 4; ++fnum) {
+            double nxo, nyo, len, nh, nb;
+            int wall, top;
+            face_geom(s, j, fnum, &nxo, &nyo, &len, &wall, &top);
+            double un = s->u[k] * nxo + s->v[k] * nyo;
+            if (wall) {
+                nh = top ? th[k] : bh[k];
+                nb = top ? tb[k] : bb[k];
+            } else {
+                int ni = i, nj = j;
+                if (fnum == 0) ni = i - 1;
+                else if (fnum == 1) ni = i + 1;
+                else if (fnum == 2) nj = j - 1;
+                else nj = j + 1;
+                if (yblock(s, nj) != yblock(s, j)) {   /* neighbour rank's halo */
+                    nh = sdhp(s, ni, nj)[k];
+                    nb = sdbp(s, ni, nj)[k];
+                } else {
+                    nh = dhp(s, ni, nj)[k];
+                    nb = dbp(s, ni, nj)[k];
+                }
+            }
+            diag = diag + (un > 0.0 ? un : 0.0) * len / area;
+            rh = rh - (un < 0.0 ? un : 0.0) * len * nh / area;
+            rb = rb - (un < 0.0 ? un : 0.0) * len * nb / area;
+        }
+        if (!(diag > 0.0 && isfinite(diag) && isfinite(rh) && isfinite(rb))) return 0;
+        dhc[k] = rh / diag;
+        dbc[k] = rb / diag;
+    }
+    return 1;
+}
+
+/* SolveMicroLUSGS(freeze_wall_closure=.true.), src/mod_ImplicitMicro.f90:366-442 */
+static int solve_micro_lusgs(or_solver *s) {
+    int nx = s->p.nx, ny = s->p.ny, n = s->nvel;
+    double *wf = dalloc((size_t)4 * n);
+    int ok = 1;
+    for (int sweep = 0; sweep < s->p.micro_sweeps && ok; ++sweep) {
+        for (int j = 0; j < ny && ok; ++j)
+            for (int i = 0; i < nx && ok; ++i) ok = relax_micro_cell(s, i, j, 0, wf, wf + 2 * n);
+        exchange_x(s->dh, s->sdh, nx, ny, n);
+        exchange_x(s->db, s->sdb, nx, ny, n);
+        for (int j = ny - 1; j >= 0 && ok; --j)
+            for (int i = nx - 1; i >= 0 && ok; --i) ok = relax_micro_cell(s, i, j, 1, wf, wf + 2 * n);
+        if (sweep + 1 < s->p.micro_sweeps) {
+            exchange_x(s->dh, s->sdh, nx, ny, n);
+            exchange_x(s->db, s->sdb, nx, ny, n);
+        }
+    }
+    free(wf);
+    return ok;
+}
+
+/* MassStatistics (totals only), src/mod_ClosedCouetteMass.f90:173-204 */
+static double mass_total(const or_solver *s) {
+    double m = 0.0, area = s->p.dx * s->p.dy;
+    for (int c = 0; c < s->ncell; ++c) m += area * s->W[4 * c];
+    return m;
+}
+
+/* ApplySteadyImplicitUpdate with the closed-Couette mass transaction,
+   src/mod_SteadyImplicit.f90:379-462, src/mod_ClosedCouetteMass.f90:243-313,
+   src/mod_R5Observe.f90:248-256.  The operator (rates, stream, physical
+   collision) must be current. */
+int or_implicit_update(or_solver *s, int *accepted, double *diag16) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell;
+    double diag[16] = {0};
+    *accepted = 0;
+    if (!s->have_M0) { s->M0 = mass_total(s); s->have_M0 = 1; }
+    memset(s->dW, 0, sizeof(double) * (size_t)ny * (nx + 2) * 4);
+    memset(s->dh, 0, sizeof(double) * (size_t)ny * (nx + 2) * n);
+    memset(s->db, 0, sizeof(double) * (size_t)ny * (nx + 2) * n);
+    memset(s->sdh, 0, sizeof(double) * (size_t)ny * (nx + 2) * n);
+    memset(s->sdb, 0, sizeof(double) * (size_t)ny * (nx + 2) * n);
+    if (!solve_macro_pr2(s)) return OR_NONFINITE;
+    /* BuildPredictedMacroState */
+    double malpha = 1.0;
+    for (;;) {
+        int ok = 1;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double t[4];
+                const double *W = s->W + 4 * cidx(s, i, j), *d = dWp(s, i, j);
+                for (int m = 0; m < 4; ++m) t[m] = W[m] + malpha * d[m];
+                ok &= is_physical(t);
+            }
+        if (ok) break;
+        malpha *= 0.5;
+        if (malpha < P->min_alpha) return OR_INADMISSIBLE;
+    }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double *d = dWp(s, i, j);
+            const double *W = s->W + 4 * cidx(s, i, j);
+            for (int m = 0; m < 4; ++m) { d[m] = malpha * d[m]; s->w_tilde[4 * cidx(s, i, j) + m] = W[m] + d[m]; }
+        }
+    if (!build_predicted_equilibrium(s)) return OR_NONFINITE;
+    if (!solve_micro_lusgs(s)) return OR_NONFINITE;
+    /* FindAdmissibleMicroAlpha without the negative-fraction test */
+    double alpha = 1.0;
+    for (;;) {
+        int ok = 1;
+        for (int j = 0; j < ny && ok; ++j)
+            for (int i = 0; i < nx && ok; ++i) {
+                size_t c = cidx(s, i, j);
+                double m[4] = {0, 0, 0, 0};
+                const double *h = s->h + c * n, *b = s->b + c * n, *dh = dhp(s, i, j), *db = dbp(s, i, j);
+                for (int k = 0; k < n; ++k) {
+                    double ch = h[k] + alpha * dh[k], cb = b[k] + alpha * db[k];
+                    if (!isfinite(ch) || !isfinite(cb)) { ok = 0; break; }
+                    m[0] += s->w[k] * ch;
+                    m[1] += s->w[k] * s->u[k] * ch;
+                    m[2] += s->w[k] * s->v[k] * ch;
+                    m[3] += s->w[k] * ((s->u[k] * s->u[k] + s->v[k] * s->v[k]) * ch + cb);
+                }
+                m[3] *= 0.5;
+                if (ok) ok = is_physical(m);
+            }
+        if (ok) break;
+        alpha *= 0.5;
+        if (alpha < P->min_alpha) return OR_INADMISSIBLE;
+    }
+    /* apply and recover W = moments(f) */
+    double mom_err = 0.0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = cidx(s, i, j);
+            double *h = s->h + c * n, *b = s->b + c * n;
+            const double *dh = dhp(s, i, j), *db = dbp(s, i, j);
+            for (int k = 0; k < n; ++k) { h[k] = h[k] + alpha * dh[k]; b[k] = b[k] + alpha * db[k]; }
+        }
+    for (int c = 0; c < nc; ++c) {
+        double m[4];
+        moment_2d(n, s->h + (size_t)c * n, s->b + (size_t)c * n, s->u, s->v, s->w, m);
+        if (!is_physical(m)) return OR_INADMISSIBLE;
+        memcpy(s->W + 4 * c, m, sizeof m);
+    }
+    diag[0] = alpha;
+    diag[1] = malpha;
+    diag[3] = mom_err;
+    /* CheckR5Policy: alpha = 1, macro alpha = 1, nCFL = 1000, one sweep pair */
+    if (P->mass_rescale) {
+        if (!(alpha == 1.0 && malpha == 1.0 && P->ncfl == 1000.0 && P->micro_sweeps == 1)) {
+            memcpy(s->last_diag, diag, sizeof diag);
+            if (diag16) memcpy(diag16, diag, sizeof diag);
+            return OR_POLICY;
+        }
+        /* RescaleClosedMass */
+        double before = mass_total(s);
+        double scale = s->M0 / before;
+        for (int c = 0; c < nc; ++c) {
+            for (int m = 0; m < 4; ++m) s->W[4 * c + m] = scale * s->W[4 * c + m];
+            double *h = s->h + (size_t)c * n, *b = s->b + (size_t)c * n;
+            for (int k = 0; k < n; ++k) { h[k] = scale * h[k]; b[k] = scale * b[k]; }
+        }
+        diag[2] = scale - 1.0;
+        diag[4] = before;
+        diag[5] = mass_total(s);
+    }
+    *accepted = 1;
+    memcpy(s->last_diag, diag, sizeof diag);
+    if (diag16) memcpy(diag16, diag, sizeof diag);
+    return OR_OK;
+}
+
+int or_iteration(or_solver *s, int *accepted, double *diag16) {
+    int st = or_operator(s);
+    if (st != OR_OK) { *accepted = 0; return st; }
+    return or_implicit_update(s, accepted, diag16);
+}
+
+void or_get_implicit(const or_solver *s, double *delta_w, double *w_tilde, double *delta_h, double *delta_b) {
+    int nx = s->p.nx, ny = s->p.ny, n = s->nvel;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = (size_t)j * nx + i, hs = (size_t)j * (nx + 2) + i + 1;
+            if (delta_w) memcpy(delta_w + 4 * c, s->dW + 4 * hs, sizeof(double) * 4);
+            if (delta_h) memcpy(delta_h + c * n, s->dh + hs * n, sizeof(double) * n);
+            if (delta_b) memcpy(delta_b + c * n, s->db + hs * n, sizeof(double) * n);
+        }
+    if (w_tilde) memcpy(w_tilde, s->w_tilde, sizeof(double) * 4 * s->ncell);
+}
+
+/* Observables: write_flow fields (src/mod_Output.f90:104-118 with
+   GetPrimary, Get_Stress_Vec, GetHeatFlux_Vec, src/mod_Tools.f90) and the
+   wall fluxes of PhysicalDiagnostic (src/mod_R5Observe.f90:101-108) of the
+   last operator. */
+void or_observables(const or_solver *s, double *fields10, double *walls16) {
+    int n = s->nvel, nx = s->p.nx, ny = s->p.ny;
+    if (fields10) {
+        for (int c = 0; c < s->ncell; ++c) {
+            double prim[4], q[2];
+            const double *h = s->h + (size_t)c * n, *b = s->b + (size_t)c * n;
+            get_primary(s->gamma, s->W + 4 * c, prim);
+            double es = 0.0, bs = 0.0, sxx = 0.0, sxy = 0.0, syy = 0.0;
+            for (int k = 0; k < n; ++k) {
+                double dvn = s->u[k] - prim[1], dvt = s->v[k] - prim[2];
+                es += s->w[k] * (dvn * dvn + dvt * dvt) * h[k];
+                bs += s->w[k] * b[k];
+            }
+            double pressure = (es + bs) / (s->p.ck + 2.0);
+            for (int k = 0; k < n; ++k) {
+                double dvn = s->u[k] - prim[1], dvt = s->v[k] - prim[2];
+                sxx += s->w[k] * dvn * dvn * h[k];
+                sxy += s->w[k] * dvn * dvt * h[k];
+                syy += s->w[k] * dvt * dvt * h[k];
+            }
+            heat_flux_vec(s, h, b, prim, q);
+            double *o = fields10 + 10 * c;
+            o[0] = prim[0]; o[1] = prim[1]; o[2] = prim[2]; o[3] = 1.0 / prim[3];
+            o[4] = 0.5 * prim[0] / prim[3];
+            o[5] = sxx - pressure; o[6] = sxy; o[7] = syy - pressure; o[8] = q[0]; o[9] = q[1];
+        }
+    }
+    if (walls16) {
+        double fl[4][4] = {{0}};
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                for (int m = 0; m < 4; ++m) {
+                    if (i == 0) fl[0][m] -= s->fxw[4 * (j * nx + nx - 1) + m] * s->p.dy / s->dt;
+                    if (i == nx - 1) fl[1][m] += s->fxw[4 * (j * nx + nx - 1) + m] * s->p.dy / s->dt;
+                    if (j == 0) fl[2][m] -= s->fyw[4 * i + m] * s->p.dx / s->dt;
+                    if (j == ny - 1) fl[3][m] += s->fyw[4 * (ny * nx + i) + m] * s->p.dx / s->dt;
+                }
+            }
+        for (int a = 0; a < 4; ++a) for (int m = 0; m < 4; ++m) walls16[a * 4 + m] = fl[a][m];
+    }
+}
+
+/* unit-level entry points */
+int or_closure_solve(const double w[4], int ck, int nvel, const double *u, const double *v, const double *wt,
+                     double *qh, double *qb, double theta[4], double jac[16], int *iters) {
+    closure_ws c;
+    closure_ws_init(&c, nvel);
+    int ok = solve_q_first(&c, w, ck, u, v, wt);
+    memcpy(qh, c.qh, sizeof(double) * nvel);
+    memcpy(qb, c.qb, sizeof(double) * nvel);
+    memcpy(theta, c.theta, sizeof c.theta);
+    memcpy(jac, c.jac, sizeof c.jac);
+    *iters = c.iters;
+    closure_ws_free(&c);
+    return ok ? OR_OK : OR_NEWTON;
+}
+
+int or_make_mke(const double w[4], int ck, double pr, int nvel, const double *u, const double *v, const double *wt,
+                double *mh, double *mb, double *kxh, double *kxb, double *kyh, double *kyb, double kappa[4],
+                double *residual) {
+    closure_ws c;
+    closure_ws_init(&c, nvel);
+    double *tmp = dalloc((size_t)2 * nvel);
+    int ok = solve_q_first(&c, w, ck, u, v, wt);
+    if (ok)
+        ok = make_mke_prepared(&c, w, ck, pr, u, v, wt, mh, mb, kxh, kxb, kyh, kyb, NULL, NULL, kappa, residual, tmp);
+    free(tmp);
+    closure_ws_free(&c);
+    return ok ? OR_OK : OR_GATE;
+}

@@ -1,0 +1,80 @@
+/*
+ * CPU restatement of the UGKS2D HighOrderImplicit pseudo-time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * it is loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py, and by nothing else.  The product library never links,
+ * loads or calls it.
+ *
+ * Every routine cites the Fortran routine of Sherlmess/UGKS2D_high_order it
+ * restates (paths relative to the reference repository root).  Indices are
+ * 0-based here: cell (i,j), i in [0,nx) periodic, j in [0,ny) between two
+ * diffuse walls; velocity k in [0,nvel) with k = iu + nu*jv (u fastest),
+ * matching src/mod_Globaldata.f90:357-361.  Storage is cell-major and
+ * velocity-contiguous: f[(j*nx+i)*nvel + k], W[(j*nx+i)*4 + m].
+ */
+#ifndef UGKS_ORACLE_H
+#define UGKS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nu, nv, ck;
+    int macro_sweeps, micro_sweeps;
+    int mass_rescale;
+    int np_y;               /* y slabs whose sweep halos are exchanged, not swept (1 = one rank) */
+    double dx, dy;
+    double cfl, ncfl, pr, omega, mu_ref;
+    double wall_ux[2];      /* bottom, top: WallUx(3:4) */
+    double wall_tn[2];      /* bottom, top wall T_native (theta = T/2) */
+    double min_alpha, moment_tol, residual_tol;
+} or_params;
+
+typedef struct or_solver or_solver;
+
+/* lifecycle */
+or_solver *or_create(const or_params *p, const double *u, const double *v, const double *w);
+void or_destroy(or_solver *s);
+int or_nthreads(int n);                     /* set OpenMP threads (returns active) */
+
+/* state: W[ncell*4], h,b[ncell*nvel] */
+int or_cold_start(or_solver *s, double rho, double ux, double tnative);
+int or_set_state(or_solver *s, const double *W, const double *h, const double *b);
+void or_get_state(const or_solver *s, double *W, double *h, double *b);
+
+/* one operator evaluation on the current state (TimeStep + Evolution + ComputeResidual) */
+int or_operator(or_solver *s);
+/* read back operator outputs; NULL pointers skip */
+void or_get_operator(const or_solver *s, double *dt, double *rate_w, double *stream_h, double *stream_b,
+                     double *coll_h, double *coll_b, double *xflux_h, double *xflux_b,
+                     double *yflux_h, double *yflux_b, double *xflux_w, double *yflux_w);
+/* residual norms of the last operator: [macro_res(4), macro_rel(4), macro_abs(4),
+   h_norm, b_norm, h_rel, b_rel, h_abs, b_abs, moment_err, mass_defect, raw_h, raw_b,
+   reference_h, reference_b] (24 values) */
+void or_get_norms(const or_solver *s, double *out24);
+/* implicit update on the current operator (ApplySteadyImplicitUpdate).  diag[16]:
+   alpha, macro_alpha, mass_scale-1, moment_err, M_before, M_after, ... */
+int or_implicit_update(or_solver *s, int *accepted, double *diag16);
+/* full iteration: operator + implicit update */
+int or_iteration(or_solver *s, int *accepted, double *diag16);
+/* observables per cell: r,u,v,T,p,sxx,sxy,syy,qx,qy (10 per cell) and wall fluxes 16 */
+void or_observables(const or_solver *s, double *fields10, double *walls16);
+/* intermediate sweep output of the last implicit update */
+void or_get_implicit(const or_solver *s, double *delta_w, double *w_tilde, double *delta_h, double *delta_b);
+
+/* --- unit-level restatements exposed for tests --- */
+void or_weno_coefficients(const double q[5], double fw[4], double coef[20]);
+int or_cell_faces9(const double *field, int nx, int ny, int i, int j,
+                   double face[12], double dn[12], double dt[12], double vol[9]);
+int or_closure_solve(const double w[4], int ck, int nvel, const double *u, const double *v, const double *wt,
+                     double *qh, double *qb, double theta[4], double jac[16], int *iters);
+int or_make_mke(const double w[4], int ck, double pr, int nvel, const double *u, const double *v,
+                const double *wt, double *mh, double *mb, double *kxh, double *kxb, double *kyh,
+                double *kyb, double kappa[4], double *residual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
