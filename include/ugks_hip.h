@@ -1,0 +1,111 @@
+/*
+ * ugks_hip.h - C ABI of the MI355X (gfx950) HighOrderImplicit pseudo-time
+ * step of the UGKS2D Couette solver.
+ *
+ * One handle holds one Couette instance resident in HBM.  Host arrays use the
+ * reference's accepted-state layout: conservative variables W(4,nx,ny) and
+ * the reduced distributions h(Nvel,nx,ny), b(Nvel,nx,ny) in Fortran column
+ * order (velocity fastest, then x, then y), i.e. C arrays W[ny][nx][4] and
+ * f[ny][nx][Nvel].  This is the layout of cell_h_store/cell_b_store
+ * (src/mod_InitializeFlow.f90:109-135) restricted to the physical cells, so
+ * a Fortran host passes cell_h_store(:,1:nx,1:ny) (contiguous copy) directly.
+ *
+ * All functions return 0 on success or a positive failure class that mirrors
+ * the reference's fail-stop classes:
+ *   1 non-finite value, 2 moment/flux gate, 3 inadmissible candidate,
+ *   4 closed-Couette policy reject, 5 Newton (discrete equilibrium) failure,
+ *   6 device/runtime error, 7 invalid argument.
+ * ugks_last_error() returns a message for the calling thread's last failure.
+ * Calls on one handle are not thread-safe; calls return after the device
+ * work they issue has completed.
+ */
+#ifndef UGKS_HIP_H
+#define UGKS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Case parameters: the Config.ini keys of &config_para/&boundary_para that
+ * the HighOrderImplicit step reads (src/mod_ImportPara.f90:141-173,911-989).
+ * The mesh is the uniform Cartesian Couette mesh on [0,1]^2. */
+typedef struct {
+    int nx, ny;               /* Nx_all, Ny_all (NP_X = NP_Y = 1) */
+    int nu, nv;               /* Nu, Nv; Nu*Nv must be 256, 1024 or 4096 */
+    int ck;                   /* internal degrees of freedom (gamma = (ck+4)/(ck+2)) */
+    int macro_sweeps;         /* Implicit_MacroSweeps (PR(2): 2) */
+    int micro_sweeps;         /* Implicit_MicroSweeps (1) */
+    int mass_rescale;         /* ClosedCouetteMassRescale */
+    double dx, dy;            /* cell size */
+    double cfl, ncfl;         /* CFL, Implicit_nCFL */
+    double pr, omega, mu_ref; /* Pr, omega (power law), CouetteMuRef */
+    double wall_ux[2];        /* WallUx(3:4): bottom, top */
+    double wall_tn[2];        /* wall T_native, bottom and top (Tw) */
+    double min_alpha, moment_tol, residual_tol; /* Implicit_MinAlpha, _MomentTol, _ResidualTol */
+} ugks_params_t;
+
+/* Create an instance.  u, v, w: the Nvel velocity table (u fastest).
+ * Replaces the allocation/initialisation of InitFlow + S2NativeInitializeState
+ * (src/mod_InitializeFlow.f90:26-57, src/mod_S2NativeAdapter.f90:192-445). */
+int ugks_create(const ugks_params_t* p, const double* u, const double* v, const double* w, void** handle);
+void ugks_destroy(void* handle);
+const char* ugks_last_error(void);
+
+/* Accepted state in/out (host arrays, layout above).  Replaces the
+ * pack_state/publish_state seam (src/mod_S2NativeAdapter.f90:153-174). */
+int ugks_set_state(void* handle, const double* W, const double* h, const double* b);
+int ugks_get_state(void* handle, double* W, double* h, double* b);
+/* Cold start: rest gas (rho, U=ux, V=0, T_native) as its discrete conservative
+ * equilibrium (S2NativeInitializeState cold branch, :276-313). */
+int ugks_cold_start(void* handle, double rho, double ux, double t_native);
+
+/* One evaluation of the physical operator on the current state: TimeStep,
+ * Evolution (S2 fluxes and targets) and ComputeResidual with the physical
+ * collision (src/mod_Solve.f90:373-421, src/mod_Evolution.f90:35-41,
+ * src/mod_Residual.f90:17-132).  Optional outputs (NULL to skip):
+ *   dt          physical time step
+ *   rate_w      (4,nx,ny)  macro residual rate
+ *   rate_h/b    (Nvel,nx,ny) streaming rate plus physical collision rate
+ *   xflux_h/b   (Nvel,nx,ny) face flux (x dt, per unit length) at the east face of each cell
+ *   yflux_h/b   (Nvel,nx,ny+1) face flux at the south face of each row (row ny = top wall)
+ */
+int ugks_evolution(void* handle, double* dt, double* rate_w, double* rate_h, double* rate_b, double* xflux_h,
+                   double* xflux_b, double* yflux_h, double* yflux_b);
+
+/* Residual diagnostics of the last operator evaluation (24 values):
+ * macro residual(4), relative(4), absolute(4), h_norm, b_norm, h_relative,
+ * b_relative, h_raw, b_raw, moment_error, mass_defect, sqrt(sum h), sqrt(sum b),
+ * reference_h, reference_b  (UpdateImplicitResidualDiagnostics,
+ * src/mod_ImplicitDiagnostics.f90:223-322) and, if walls16 != NULL, the wall
+ * flux sums of PhysicalDiagnostic (src/mod_R5Observe.f90:101-108). */
+int ugks_norms(void* handle, double* out24, double* walls16);
+
+/* One accepted-state update ApplySteadyImplicitUpdate with the closed-Couette
+ * transaction (src/mod_SteadyImplicit.f90:379-462): PR(2), predicted
+ * equilibrium, one frozen-wall LU-SGS pair, admissibility, macro recovery,
+ * policy check and mass rescale.  Evaluates the operator first unless the
+ * last ugks_evolution was on the current state.  On rejection the state is
+ * left unchanged and *accepted = 0.  diag16: [alpha, macro_alpha, scale-1,
+ * moment_error, M_before, M_after, M0, dt, dtp, status bits, ...]. */
+int ugks_implicit_step(void* handle, int* accepted, double* diag16);
+
+/* Run n iterations (operator + update each), stopping early on rejection.
+ * Returns the number of accepted iterations in *done. */
+int ugks_run(void* handle, int n, int* done);
+
+/* Observables of the current state per cell (10 values: rho, U, V, T_native,
+ * p, sigma_xx, sigma_xy, sigma_yy, q_x, q_y; write_flow formulas,
+ * src/mod_Output.f90:104-118 with src/mod_Tools.f90:209-235,355-373,409-432). */
+int ugks_observables(void* handle, double* fields10);
+
+/* Device-resident access for hosts that keep their own device buffers
+ * (HIP device pointers, same layouts). */
+int ugks_set_state_device(void* handle, const double* W, const double* h, const double* b);
+int ugks_get_state_device(void* handle, double* W, double* h, double* b);
+/* Synchronize the instance's stream. */
+int ugks_sync(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

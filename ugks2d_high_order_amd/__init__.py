@@ -1,0 +1,224 @@
+"""MI355X (gfx950) HighOrderImplicit pseudo-time step of the UGKS2D Couette solver.
+
+The compute path is the HIP library libugks_hip.so (C ABI in
+include/ugks_hip.h); this package is the host-side mirror of the reference's
+Solve_UGKS loop (src/mod_Solve.f90:84-293) over that ABI.  There is no CPU
+fallback: every entry point fails loudly when the library or a HIP device is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import config, velocity  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libugks_hip.so")
+
+EXPORTS = [
+    "ugks_create", "ugks_destroy", "ugks_last_error", "ugks_set_state", "ugks_get_state", "ugks_cold_start",
+    "ugks_evolution", "ugks_norms", "ugks_implicit_step", "ugks_run", "ugks_observables",
+    "ugks_set_state_device", "ugks_get_state_device", "ugks_sync",
+]
+
+FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
+           5: "Newton failure", 6: "device error", 7: "invalid argument"}
+
+
+class UGKSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{FAILURE.get(code, 'error')} ({code}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nu", C.c_int), ("nv", C.c_int), ("ck", C.c_int),
+        ("macro_sweeps", C.c_int), ("micro_sweeps", C.c_int), ("mass_rescale", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("ncfl", C.c_double),
+        ("pr", C.c_double), ("omega", C.c_double), ("mu_ref", C.c_double),
+        ("wall_ux", C.c_double * 2), ("wall_tn", C.c_double * 2),
+        ("min_alpha", C.c_double), ("moment_tol", C.c_double), ("residual_tol", C.c_double),
+    ]
+
+
+_lib = None
+_D = C.POINTER(C.c_double)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libugks_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with python -m ugks2d_high_order_amd.build")
+    L = C.CDLL(path)
+    L.ugks_create.argtypes = [C.POINTER(Params), _D, _D, _D, C.POINTER(C.c_void_p)]
+    L.ugks_destroy.argtypes = [C.c_void_p]
+    L.ugks_destroy.restype = None
+    L.ugks_last_error.restype = C.c_char_p
+    L.ugks_set_state.argtypes = [C.c_void_p, _D, _D, _D]
+    L.ugks_get_state.argtypes = [C.c_void_p, _D, _D, _D]
+    L.ugks_set_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ugks_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ugks_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.ugks_evolution.argtypes = [C.c_void_p] + [_D] * 8
+    L.ugks_norms.argtypes = [C.c_void_p, _D, _D]
+    L.ugks_implicit_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+    L.ugks_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.ugks_observables.argtypes = [C.c_void_p, _D]
+    L.ugks_sync.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _check(rc: int):
+    if rc:
+        raise UGKSError(rc, load_library().ugks_last_error().decode(errors="replace"))
+
+
+def make_params(case) -> Params:
+    p = Params()
+    p.nx, p.ny, p.nu, p.nv, p.ck = case.nx, case.ny, case.nu, case.nv, case.ck
+    p.macro_sweeps, p.micro_sweeps, p.mass_rescale = case.macro_sweeps, case.micro_sweeps, int(case.mass_rescale)
+    p.dx, p.dy, p.cfl, p.ncfl = case.dx, case.dy, case.cfl, case.ncfl
+    p.pr, p.omega, p.mu_ref = case.pr, case.omega, case.mu_ref
+    p.wall_ux[0], p.wall_ux[1] = case.wall_ux
+    p.wall_tn[0], p.wall_tn[1] = case.wall_tn
+    p.min_alpha, p.moment_tol, p.residual_tol = case.min_alpha, case.moment_tol, case.residual_tol
+    return p
+
+
+class UGKS:
+    """One Couette instance resident on the GPU.
+
+    Method names follow the reference's entry points: evolution() is
+    Evolution()+ComputeResidual() (src/mod_Evolution.f90:35-41,
+    src/mod_Residual.f90:17-24), implicit_step() is
+    ApplySteadyImplicitUpdate(accepted) (src/mod_SteadyImplicit.f90:379-396).
+    """
+
+    def __init__(self, case, u=None, v=None, w=None):
+        self.case = case
+        if u is None:
+            u, v, w = velocity.table(case.table, case.table_scale)
+        self.u = np.ascontiguousarray(u, dtype=np.float64)
+        self.v = np.ascontiguousarray(v, dtype=np.float64)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        assert self.u.size == case.nu * case.nv
+        self.nvel = self.u.size
+        self.ncell = case.nx * case.ny
+        L = load_library()
+        self._p = make_params(case)
+        h = C.c_void_p()
+        _check(L.ugks_create(C.byref(self._p), _ptr(self.u), _ptr(self.v), _ptr(self.w), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().ugks_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cold_start(self, rho: float = 1.0, ux: float = 0.0, t_native: float | None = None):
+        t = self.case.t_init if t_native is None else t_native
+        _check(load_library().ugks_cold_start(self.handle, rho, ux, t))
+
+    def set_state(self, W, h, b):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.ncell, 4)
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(self.ncell, self.nvel)
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(self.ncell, self.nvel)
+        _check(load_library().ugks_set_state(self.handle, _ptr(W), _ptr(h), _ptr(b)))
+
+    def state(self):
+        W = np.zeros((self.ncell, 4)); h = np.zeros((self.ncell, self.nvel)); b = np.zeros((self.ncell, self.nvel))
+        _check(load_library().ugks_get_state(self.handle, _ptr(W), _ptr(h), _ptr(b)))
+        return W, h, b
+
+    def evolution(self, outputs: bool = True):
+        """Operator on the current state; returns dt, rates and face fluxes."""
+        nc, n, nx, ny = self.ncell, self.nvel, self.case.nx, self.case.ny
+        if not outputs:
+            _check(load_library().ugks_evolution(self.handle, *([None] * 8)))
+            return None
+        out = dict(dt=np.zeros(1), rate_w=np.zeros((nc, 4)), rate_h=np.zeros((nc, n)), rate_b=np.zeros((nc, n)),
+                   xflux_h=np.zeros((nc, n)), xflux_b=np.zeros((nc, n)),
+                   yflux_h=np.zeros(((ny + 1) * nx, n)), yflux_b=np.zeros(((ny + 1) * nx, n)))
+        keys = ["dt", "rate_w", "rate_h", "rate_b", "xflux_h", "xflux_b", "yflux_h", "yflux_b"]
+        _check(load_library().ugks_evolution(self.handle, *[_ptr(out[k]) for k in keys]))
+        out["dt"] = float(out["dt"][0])
+        return out
+
+    def norms(self):
+        o = np.zeros(24); w = np.zeros(16)
+        _check(load_library().ugks_norms(self.handle, _ptr(o), _ptr(w)))
+        return o, w
+
+    def implicit_step(self):
+        acc = C.c_int(0)
+        d = np.zeros(16)
+        _check(load_library().ugks_implicit_step(self.handle, C.byref(acc), _ptr(d)))
+        return bool(acc.value), d
+
+    def run(self, n: int) -> int:
+        done = C.c_int(0)
+        _check(load_library().ugks_run(self.handle, int(n), C.byref(done)))
+        return done.value
+
+    def observables(self):
+        f = np.zeros((self.ncell, 10))
+        _check(load_library().ugks_observables(self.handle, _ptr(f)))
+        return f
+
+    def sync(self):
+        _check(load_library().ugks_sync(self.handle))
+
+
+def solve_ugks(case, max_iter: int | None = None, streak_needed: int = 5, callback=None, solver: UGKS | None = None):
+    """Pseudo-time loop of the HighOrderImplicit entry (Solve_UGKS with the
+    closed-Couette R5/R6 acceptance, src/mod_Solve.f90:84-293,
+    src/mod_R5Observe.f90:287-320): each iteration evaluates the operator of
+    the accepted state (its residuals are that state's verification), stops
+    once `streak_needed` consecutive accepted states pass the gate
+    (macro residuals and h/b norms <= Implicit_ResidualTol), and otherwise
+    applies one implicit update.  The operator of a state is computed once
+    and serves both its verification and the next update."""
+    s = solver or UGKS(case)
+    if solver is None:
+        s.cold_start()
+    n = case.max_iter if max_iter is None else max_iter
+    streak, it = 0, 0
+    history = []
+    while it < n:
+        s.evolution(outputs=False)
+        norms, walls = s.norms()
+        if it > 0:
+            ok = bool(np.all(norms[0:4] <= case.residual_tol) and norms[12] <= case.residual_tol
+                      and norms[13] <= case.residual_tol)
+            streak = streak + 1 if ok else 0
+            history.append(dict(iter=it, norms=norms, walls=walls, streak=streak))
+            if callback:
+                callback(it, norms, walls)
+            if streak >= streak_needed:
+                break
+        acc, d = s.implicit_step()
+        if not acc:
+            raise UGKSError(4, "update rejected")
+        it += 1
+    return s, history

@@ -1,0 +1,283 @@
+// Device building blocks of the UGKS2D HighOrderImplicit step for gfx950.
+//
+// Everything is FP64.  Velocity-space work is spread over the threads of a
+// workgroup: thread t owns velocities k = t + NT*r, r < KPT, so the loads of a
+// wavefront are contiguous in k.  Moments are deterministic: each thread sums
+// its own velocities in order, a wavefront butterfly (__shfl_xor) combines
+// lanes, and the per-wave partials are added in wave order from LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define UGKS_PI 3.14159265358979323846264338327950288
+#define UGKS_GATE 2.0e-12
+#define UGKS_NEWTON_TOL 5.0e-13
+#define UGKS_DBL_EPS 2.220446049250313080847e-16
+#define UGKS_DBL_MIN 2.2250738585072013830903e-308
+
+// error bits reported through the device status word
+enum : unsigned {
+    ST_NEWTON = 1u,        // discrete-equilibrium Newton failed
+    ST_GATE = 2u,          // a moment gate (2e-12) failed
+    ST_NONFINITE = 4u,     // non-finite value
+    ST_WALL = 8u,          // diffuse wall closure failed
+    ST_MACRO_ALPHA = 16u,  // predicted macro state needed alpha < 1
+    ST_MICRO_ALPHA = 32u,  // micro candidate not admissible at alpha = 1
+    ST_PROJECT = 64u,      // moment projection failed
+    ST_SOLVE = 128u,       // singular 4x4 system
+};
+
+struct Consts {
+    double gnode[3];
+    double gw[3];
+    double cmap[4][5][5];      // WENO candidate maps [candidate][sample][degree]
+    double wallmap[5][5][5];   // one-sided quartic maps, samples at offsets -sh..4-sh
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+    return x;
+}
+__device__ inline double wave_max(double x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = fmax(x, __shfl_xor(x, m, 64));
+    return x;
+}
+
+// Sum R values over the whole workgroup; every thread gets the totals.
+// lds must hold (blockDim.x/64)*R doubles.
+template <int R>
+__device__ inline void block_sum(double* v, double* lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = wave_sum(v[r]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[wid * R + r] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += lds[w * R + r];
+        v[r] = s;
+    }
+}
+
+template <int R>
+__device__ inline void block_max(double* v, double* lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = wave_max(v[r]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[wid * R + r] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double s = lds[r];
+        for (int w = 1; w < nw; ++w) s = fmax(s, lds[w * R + r]);
+        v[r] = s;
+    }
+}
+
+// moments (rho, rho U, rho V, rho E) contribution of one velocity
+__device__ inline void mom_add(double m[4], double w, double u, double v, double h, double b) {
+    m[0] += w * h;
+    m[1] += w * u * h;
+    m[2] += w * v * h;
+    m[3] += 0.5 * w * ((u * u + v * v) * h + b);
+}
+
+// ---------------------------------------------------------------------------
+// small dense solves (uniform across the workgroup)
+// ---------------------------------------------------------------------------
+// Gaussian elimination with partial pivoting; m column-major (m[c*4+r]).
+__device__ inline bool solve4(const double* m, const double* rhs, double* x) {
+    double a[4][4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[r][c] = m[c * 4 + r];
+        b[r] = rhs[r];
+    }
+    for (int p = 0; p < 3; ++p) {
+        int row = p;
+        double best = fabs(a[p][p]);
+        for (int r = p + 1; r < 4; ++r)
+            if (fabs(a[r][p]) > best) { best = fabs(a[r][p]); row = r; }
+        if (!(fabs(a[row][p]) > 100.0 * UGKS_DBL_MIN) || !isfinite(a[row][p])) return false;
+        if (row != p) {
+            for (int c = 0; c < 4; ++c) { double t = a[p][c]; a[p][c] = a[row][c]; a[row][c] = t; }
+            double t = b[p]; b[p] = b[row]; b[row] = t;
+        }
+        for (int r = p + 1; r < 4; ++r) {
+            double f = a[r][p] / a[p][p];
+            for (int c = p; c < 4; ++c) a[r][c] -= f * a[p][c];
+            b[r] -= f * b[p];
+        }
+    }
+    if (!(fabs(a[3][3]) > 100.0 * UGKS_DBL_MIN) || !isfinite(a[3][3])) return false;
+    for (int r = 3; r >= 0; --r) {
+        double s = 0.0;
+        for (int c = r + 1; c < 4; ++c) s += a[r][c] * x[c];
+        x[r] = (b[r] - s) / a[r][r];
+    }
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+// Gauss-Jordan with partial pivoting and a relative pivot floor (the
+// moment-projection solve); m column-major.
+__device__ inline bool solve4_gj(const double* m, const double* rhs, double* x) {
+    double a[4][5], scale = 0.0;
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) { a[r][c] = m[c * 4 + r]; scale = fmax(scale, fabs(a[r][c])); }
+        a[r][4] = rhs[r];
+    }
+    scale = fmax(scale, UGKS_DBL_MIN);
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
+        if (!(fabs(a[piv][col]) > 1000.0 * UGKS_DBL_EPS * scale)) return false;
+        if (piv != col)
+            for (int c = 0; c < 5; ++c) { double t = a[col][c]; a[col][c] = a[piv][c]; a[piv][c] = t; }
+        double d = a[col][col];
+        for (int c = 0; c < 5; ++c) a[col][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            double f = a[r][col];
+            for (int c = 0; c < 5; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r) x[r] = a[r][4];
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+// ---------------------------------------------------------------------------
+// gas model
+// ---------------------------------------------------------------------------
+struct Gas {
+    int ck;
+    double pr, omega, mu_ref, gamma;
+};
+
+// collision time from conserved W (power-law viscosity): tau = mu/p
+__device__ inline double tau_of_w(const Gas& g, const double w[4]) {
+    double p = 2.0 / (double)(g.ck + 2) * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+    double lam = 0.5 * w[0] / p;
+    double mu = g.mu_ref * pow(1.0 / lam, g.omega);
+    return mu / (0.5 * w[0] / lam);
+}
+
+// primitive (rho, U, V, lambda=1/T_native) as GetPrimary does
+__device__ inline void primary(double gamma, const double w[4], double prim[4]) {
+    double denom = w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0];
+    prim[0] = w[0];
+    prim[1] = w[1] / w[0];
+    prim[2] = w[2] / w[0];
+    prim[3] = 0.5 * w[0] / ((gamma - 1.0) * denom);
+}
+
+__device__ inline double tau_of_prim(const Gas& g, const double prim[4]) {
+    double mu = g.mu_ref * pow(1.0 / prim[3], g.omega);
+    return mu / (0.5 * prim[0] / prim[3]);
+}
+
+__device__ inline bool physical_w(const double w[4]) {
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || !(w[0] > 0.0)) return false;
+    double e = w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0];
+    return isfinite(e) && e > 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// WENO5-AO and the one-sided wall quartic
+// ---------------------------------------------------------------------------
+__device__ inline void weno_weights(const double q[5], double fw[4]) {
+    const double g0 = 0.01125, g1 = 0.1275, g2 = 0.01125, g3 = 0.85;
+    double t, b0, b1, b2, b3;
+    t = q[0] - 2.0 * q[1] + q[2]; b0 = 13.0 / 12.0 * t * t;
+    t = q[0] - 4.0 * q[1] + 3.0 * q[2]; b0 += 0.25 * t * t;
+    t = q[1] - 2.0 * q[2] + q[3]; b1 = 13.0 / 12.0 * t * t;
+    t = q[1] - q[3]; b1 += 0.25 * t * t;
+    t = q[2] - 2.0 * q[3] + q[4]; b2 = 13.0 / 12.0 * t * t;
+    t = 3.0 * q[2] - 4.0 * q[3] + q[4]; b2 += 0.25 * t * t;
+    b3 = (231153.0 * q[2] * q[2] + 104963.0 * q[1] * q[1] + 6908.0 * q[0] * q[0] - 38947.0 * q[0] * q[3] +
+          104963.0 * q[3] * q[3] + q[1] * (-51001.0 * q[0] + 179098.0 * q[3] - 38947.0 * q[4]) -
+          3.0 * q[2] * (99692.0 * q[1] - 22641.0 * q[0] + 99692.0 * q[3] - 22641.0 * q[4]) +
+          8209.0 * q[0] * q[4] - 51001.0 * q[3] * q[4] + 6908.0 * q[4] * q[4]) /
+         5040.0;
+    double tau = (fabs(b3 - b0) + fabs(b3 - b1) + fabs(b3 - b2)) / 3.0;
+    double r0 = tau / (b0 + 1.0e-8), r1 = tau / (b1 + 1.0e-8), r2 = tau / (b2 + 1.0e-8), r3 = tau / (b3 + 1.0e-8);
+    double a0 = g0 * (1.0 + r0 * r0), a1 = g1 * (1.0 + r1 * r1), a2 = g2 * (1.0 + r2 * r2), a3 = g3 * (1.0 + r3 * r3);
+    double sa = a0 + a1 + a2 + a3;
+    double o0 = a0 / sa, o1 = a1 / sa, o2 = a2 / sa, o3 = a3 / sa;
+    fw[3] = o3 / g3;
+    fw[0] = o0 - fw[3] * g0;
+    fw[1] = o1 - fw[3] * g1;
+    fw[2] = o2 - fw[3] * g2;
+}
+
+// polynomial coefficients of the four candidates: quadratic candidates c<3
+// (degrees 0..2) and the quartic (degrees 0..4)
+__device__ inline void weno_polys(const Consts& K, const double q[5], double cq[3][3], double c4[5]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int t = c; t < c + 3; ++t) s += K.cmap[c][t][d] * q[t];
+            cq[c][d] = s;
+        }
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) s += K.cmap[3][t][d] * q[t];
+        c4[d] = s;
+    }
+}
+
+// value and derivative of the weighted reconstruction at xi
+__device__ inline void weno_at(const double fw[4], const double cq[3][3], const double c4[5], double xi,
+                               double* val, double* der) {
+    double v = 0.0, d = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v += fw[c] * (cq[c][0] + xi * (cq[c][1] + xi * cq[c][2]));
+        d += fw[c] * (cq[c][1] + 2.0 * xi * cq[c][2]);
+    }
+    v += fw[3] * (c4[0] + xi * (c4[1] + xi * (c4[2] + xi * (c4[3] + xi * c4[4]))));
+    d += fw[3] * (c4[1] + xi * (2.0 * c4[2] + xi * (3.0 * c4[3] + xi * 4.0 * c4[4])));
+    *val = v;
+    if (der) *der = d;
+}
+
+__device__ inline double weno_val(const double fw[4], const double cq[3][3], const double c4[5], double xi) {
+    double v;
+    weno_at(fw, cq, c4, xi, &v, nullptr);
+    return v;
+}
+
+// one-sided quartic through five samples, value and derivative at xi
+__device__ inline void wall_at(const Consts& K, int sh, const double q[5], double xi, double* val, double* der) {
+    double c[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) s += K.wallmap[sh][t][d] * q[t];
+        c[d] = s;
+    }
+    *val = c[0] + xi * (c[1] + xi * (c[2] + xi * (c[3] + xi * c[4])));
+    if (der) *der = c[1] + xi * (2.0 * c[2] + xi * (3.0 * c[3] + xi * 4.0 * c[4]));
+}

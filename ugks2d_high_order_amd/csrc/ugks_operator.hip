@@ -1,0 +1,1048 @@
+// S2 high-order operator of the HighOrderImplicit step on gfx950:
+// reconstruction, UGKS face and wall fluxes with the discrete-equilibrium
+// Newton closure, volume targets, physical Shakhov collision, streaming
+// rates and the residual diagnostics.  Restates, in the reference:
+//   compute_flux / ugks_node / limit_cell_outputs   src/mod_S2D1.f90:315-345,421-521,632-959
+//   s2w2_wall_node                                  src/mod_S2MultiscaleWall.f90:45-184
+//   solve_q_first / closure directions               src/mod_CorrectedWF2DClosure.f90:60-144,330-432
+//   build_targets / make_mke                        src/mod_S2D1.f90:99-151,1246-1300
+//   residual assembly and physical collision        src/mod_Residual.f90:17-132,
+//                                                   src/mod_S2NativeAdapter.f90:735-759
+//   UpdateImplicitResidualDiagnostics               src/mod_ImplicitDiagnostics.f90:223-322
+//   TimeStep                                         src/mod_Solve.f90:373-421
+#include "ugks_recon.h"
+#include "ugks_state.h"
+
+__constant__ Consts KC;
+
+// ===========================================================================
+// time step (one workgroup)
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vmax) {
+    __shared__ double lds[4];
+    double tmin = 1e300;
+    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
+        double prim[4];
+        primary(D.gas.gamma, D.W + 4 * c, prim);
+        double sos = sqrt(0.5 * D.gas.gamma / prim[3]);
+        double pu = fmax(umax, fabs(prim[1]) + sos), pv = fmax(vmax, fabs(prim[2]) + sos);
+        double dtl = D.cfl * (D.dx * D.dy) / (D.dy * pu + D.dx * pv);
+        tmin = fmin(tmin, dtl);
+    }
+    double v[1] = {-tmin};
+    block_max<1>(v, lds);
+    if (threadIdx.x == 0) {
+        D.scal[SC_DT] = -v[0];
+        D.scal[SC_DTP] = -v[0] * D.ncfl / D.cfl;
+    }
+}
+
+// ===========================================================================
+// macro (W) reconstruction: face slopes and volume nodes, unlimited
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_wrecon(Dev D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * D.ncell) return;
+    int c = t >> 2, m = t & 3, i = c % D.nx, j = c / D.nx;
+    int rb, sh;
+    bool near;
+    stencil_rows(j, D.ny, &rb, &near, &sh);
+    double blk[5][5];
+    for (int r = 0; r < 5; ++r)
+        for (int s = 0; s < 5; ++s) {
+            int col = i + s - 2;
+            col = col < 0 ? col + D.nx : (col >= D.nx ? col - D.nx : col);
+            blk[r][s] = D.W[4 * ((rb + r) * D.nx + col) + m];
+        }
+    FirstLayer F;
+    first_layer(KC, blk, near, sh, F);
+    double* o = D.ws + (size_t)(c * 4 + m) * 24;
+    for (int s = 0; s < 4; ++s) {
+        double d[3][3];
+        side_data(KC, F, near, sh, s, d);
+        for (int a = 0; a < 3; ++a) { o[(s * 3 + a) * 2 + 0] = d[a][1]; o[(s * 3 + a) * 2 + 1] = d[a][2]; }
+    }
+    double vol[3][3];
+    vol_data(KC, F, near, sh, vol);
+    double* ov = D.wv + (size_t)(c * 4 + m) * 9;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) ov[a * 3 + b] = vol[a][b];
+}
+
+// ===========================================================================
+// micro reconstruction: one thread per (cell, velocity), h and b together
+// ===========================================================================
+struct SideOut {
+    double d[3][3];   // node x {value, dn, dt}
+};
+
+__device__ inline void recon_field(const double* F, int nx, int nvel, int i, int rb, bool near, int sh, int k,
+                                   int xs, int ys, int wsd, double fval[4][3], double vol[3][3], SideOut& xo,
+                                   SideOut& yo, SideOut& wo) {
+    double blk[5][5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            int col = i + s - 2;
+            col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+            blk[r][s] = F[(size_t)((rb + r) * nx + col) * nvel + k];
+        }
+    FirstLayer L;
+    first_layer(KC, blk, near, sh, L);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        double d[3][3];
+        side_data(KC, L, near, sh, s, d);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            fval[s][a] = d[a][0];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (s == xs) xo.d[a][q] = d[a][q];
+                if (s == ys) yo.d[a][q] = d[a][q];
+                if (s == wsd) wo.d[a][q] = d[a][q];
+            }
+        }
+    }
+    vol_data(KC, L, near, sh, vol);
+}
+
+__device__ inline double theta_of(double m, double fval[4][3], double vol[3][3], double theta) {
+    const double keep = 1.0 - 64.0 * UGKS_DBL_EPS;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (fval[s][a] < 0.0) theta = fmin(theta, keep * m / (m - fval[s][a]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (vol[a][b] < 0.0) theta = fmin(theta, keep * m / (m - vol[a][b]));
+    return theta;
+}
+
+// write one limited side for h (q 0..2) and b (q 3..5) into a face record
+__device__ inline void store_side(double* rec, int nvel, int k, double theta, double hm, double bm,
+                                  const SideOut& sh_, const SideOut& sb_, double sn, double st) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double* p = rec + (size_t)(a * 6) * nvel + k;
+        p[0 * (size_t)nvel] = hm + theta * (sh_.d[a][0] - hm);
+        p[1 * (size_t)nvel] = theta * sh_.d[a][1] / sn;
+        p[2 * (size_t)nvel] = theta * sh_.d[a][2] / st;
+        p[3 * (size_t)nvel] = bm + theta * (sb_.d[a][0] - bm);
+        p[4 * (size_t)nvel] = theta * sb_.d[a][1] / sn;
+        p[5 * (size_t)nvel] = theta * sb_.d[a][2] / st;
+    }
+}
+
+__global__ void __launch_bounds__(128) k_recon(Dev D) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= D.nvel) return;
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    const int i = c % nx, j = c / nx;
+    int rb, sh;
+    bool near;
+    stencil_rows(j, ny, &rb, &near, &sh);
+    const double uk = D.u[k], vk = D.v[k];
+    const int xs = uk >= 0.0 ? 1 : 0;
+    const int ys = vk >= 0.0 ? 3 : 2;
+    const int wsd = j == 0 ? 2 : (j == ny - 1 ? 3 : -1);
+    double fv[4][3], vol[3][3];
+    SideOut hx, hy, hw, bx, by, bw;
+    const double hm = D.h[(size_t)c * nvel + k], bm = D.b[(size_t)c * nvel + k];
+    double theta = 1.0;
+    recon_field(D.h, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, hx, hy, hw);
+    theta = theta_of(hm, fv, vol, theta);
+    recon_field(D.b, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, bx, by, bw);
+    theta = theta_of(bm, fv, vol, theta);
+    if (hm <= 0.0 || bm <= 0.0) theta = 0.0;
+    theta = fmax(0.0, fmin(1.0, theta));
+    // x face: upwind side of this cell
+    {
+        int f = xs == 1 ? c : j * nx + (i == 0 ? nx - 1 : i - 1);
+        store_side(D.xf + (size_t)f * 18 * nvel, nvel, k, theta, hm, bm, hx, bx, D.dx, D.dy);
+    }
+    // interior y faces
+    if (ys == 3 && j < ny - 1)
+        store_side(D.yf + (size_t)c * 18 * nvel, nvel, k, theta, hm, bm, hy, by, D.dy, D.dx);
+    if (ys == 2 && j > 0)
+        store_side(D.yf + (size_t)(c - nx) * 18 * nvel, nvel, k, theta, hm, bm, hy, by, D.dy, D.dx);
+    // wall rows: the wall side at every velocity
+    if (j == 0) store_side(D.wb + (size_t)i * 18 * nvel, nvel, k, theta, hm, bm, hw, bw, D.dy, D.dx);
+    if (j == ny - 1) store_side(D.wt + (size_t)i * 18 * nvel, nvel, k, theta, hm, bm, hw, bw, D.dy, D.dx);
+}
+
+// ===========================================================================
+// velocity-parallel closure machinery
+// ===========================================================================
+template <int KPT, int NT>
+struct VelR {
+    double u[KPT], v[KPT], w[KPT];
+};
+
+template <int KPT, int NT>
+__device__ inline void load_vel(const Dev& D, VelR<KPT, NT>& V) {
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        V.u[r] = D.u[k]; V.v[r] = D.v[k]; V.w[r] = D.w[k];
+    }
+}
+
+// q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q, with the moments of q and
+// of dq/dtheta_a (Jacobian columns): red[0..3] moments, red[4+4a..] column a
+template <int KPT, int NT>
+__device__ inline void q_pass(int ck, const double th[4], const VelR<KPT, NT>& V, double qh[KPT], double red[20],
+                              double* lds) {
+#pragma unroll
+    for (int a = 0; a < 20; ++a) red[a] = 0.0;
+    const double r0 = th[0], T = th[3];
+    const double pre = r0 / (UGKS_PI * T);
+    const double i0 = 1.0 / r0, iT = 1.0 / T;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        double du = V.u[r] - th[1], dv = V.v[r] - th[2], c2 = du * du + dv * dv;
+        double q = pre * exp(-c2 / T);
+        qh[r] = q;
+        double qb = 0.5 * (double)ck * T * q;
+        double l1 = 2.0 * du / T, l2 = 2.0 * dv / T, l3 = -1.0 / T + c2 / (T * T);
+        double lb3 = l3 + iT;
+        mom_add(red + 0, V.w[r], V.u[r], V.v[r], q, qb);
+        mom_add(red + 4, V.w[r], V.u[r], V.v[r], q * i0, qb * i0);
+        mom_add(red + 8, V.w[r], V.u[r], V.v[r], q * l1, qb * l1);
+        mom_add(red + 12, V.w[r], V.u[r], V.v[r], q * l2, qb * l2);
+        mom_add(red + 16, V.w[r], V.u[r], V.v[r], q * l3, qb * lb3);
+    }
+    block_sum<20>(red, lds);
+}
+
+__device__ inline double scaled_res(const double r[4], const double w[4]) {
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) m = fmax(m, fabs(r[a]) / fmax(1.0, fabs(w[a])));
+    return m;
+}
+
+// Newton solve for the discrete conservative equilibrium with moments w
+// (halving line search down to 2^-20, at most 12 steps); on success th, J and
+// the owned q values describe the solution.
+template <int KPT, int NT>
+__device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, NT>& V, double th[4], double J[16],
+                                     double qh[KPT], double* lds) {
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || !(w[0] > 0.0)) return false;
+    th[0] = w[0];
+    th[1] = w[1] / w[0];
+    th[2] = w[2] / w[0];
+    th[3] = 4.0 * (w[3] / w[0] - 0.5 * (th[1] * th[1] + th[2] * th[2])) / (double)(ck + 2);
+    if (!(th[3] > 0.0) || !isfinite(th[3])) return false;
+    double red[20], res[4];
+    q_pass<KPT, NT>(ck, th, V, qh, red, lds);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
+    double nrm = scaled_res(res, w);
+    for (int it = 0; it < 12; ++it) {
+        if (nrm <= UGKS_NEWTON_TOL) break;
+        double step[4], mres[4] = {-res[0], -res[1], -res[2], -res[3]};
+        if (!solve4(J, mres, step)) return false;
+        double alpha = 1.0;
+        bool acc = false;
+        while (alpha >= 9.5367431640625e-07) {
+            double nt[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) nt[a] = th[a] + alpha * step[a];
+            if (nt[0] > 0.0 && nt[3] > 0.0 && isfinite(nt[0]) && isfinite(nt[1]) && isfinite(nt[2]) &&
+                isfinite(nt[3])) {
+                double qt[KPT];
+                q_pass<KPT, NT>(ck, nt, V, qt, red, lds);
+                bool fin = true;
+#pragma unroll
+                for (int a = 0; a < 20; ++a) fin = fin && isfinite(red[a]);
+                if (fin) {
+                    double nres[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) nres[a] = red[a] - w[a];
+                    double nn = scaled_res(nres, w);
+                    if (nn < nrm || nn <= UGKS_NEWTON_TOL) {
+                        acc = true;
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) { th[a] = nt[a]; res[a] = nres[a]; }
+#pragma unroll
+                        for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
+#pragma unroll
+                        for (int r = 0; r < KPT; ++r) qh[r] = qt[r];
+                        nrm = nn;
+                    }
+                }
+                if (acc) break;
+            }
+            alpha *= 0.5;
+        }
+        if (!acc) return false;
+    }
+    if (!(nrm <= UGKS_NEWTON_TOL)) return false;
+    int pos = 1;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) pos &= (qh[r] > 0.0 && isfinite(qh[r])) ? 1 : 0;
+    return __syncthreads_and(pos) != 0;
+}
+
+// dq/dtheta_a of the owned velocity r, h and b parts
+__device__ inline void qtheta(int ck, const double th[4], double u, double v, double q, double qt_h[4],
+                             double qt_b[4]) {
+    double T = th[3];
+    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+    double qb = 0.5 * (double)ck * T * q;
+    double l0 = 1.0 / th[0], l1 = 2.0 * du / T, l2 = 2.0 * dv / T, l3 = -1.0 / T + c2 / (T * T);
+    qt_h[0] = q * l0; qt_h[1] = q * l1; qt_h[2] = q * l2; qt_h[3] = q * l3;
+    qt_b[0] = qb * l0; qt_b[1] = qb * l1; qt_b[2] = qb * l2; qt_b[3] = qb * (l3 + 1.0 / T);
+}
+
+// S2 primitive (theta_gas = p/rho)
+__device__ inline bool s2_prim(int ck, const double w[4], double* rho, double* ux, double* vy, double* thg) {
+    *rho = w[0];
+    if (!(w[0] > 0.0)) return false;
+    *ux = w[1] / w[0];
+    *vy = w[2] / w[0];
+    double eint = w[3] - 0.5 * w[0] * ((*ux) * (*ux) + (*vy) * (*vy));
+    if (!(eint > 0.0)) return false;
+    double p = (2.0 / (double)(ck + 2)) * eint;
+    *thg = p / w[0];
+    return isfinite(*thg) && *thg > 0.0;
+}
+
+// ===========================================================================
+// UGKS interior face flux: one workgroup per face, three Gauss nodes
+// ===========================================================================
+template <int KPT, int NT>
+__device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int axis, double dt, const double f0h[KPT],
+                                     const double f0b[KPT], const double fnh[KPT], const double fnb[KPT],
+                                     const double fth[KPT], const double ftb[KPT], const double mnl[4],
+                                     const double mnr[4], const double mtg[4], double gwa, double acc_h[KPT],
+                                     double acc_b[KPT], double fw_acc[4], double* lds) {
+    const int ck = D.gas.ck;
+    double w0[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) mom_add(w0, V.w[r], V.u[r], V.v[r], f0h[r], f0b[r]);
+    block_sum<4>(w0, lds);
+    double th[4], J[16], qh[KPT];
+    if (!closure_solve<KPT, NT>(ck, w0, V, th, J, qh, lds)) return ST_NEWTON;
+    double rho, ux, vy, thg;
+    if (!s2_prim(ck, w0, &rho, &ux, &vy, &thg)) return ST_NONFINITE;
+    const double lam = 0.5 / thg;
+    const double coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho;
+    // analytic Maxwellian of w0 (T_native from conserved variables)
+    const double kin = 0.5 * (ux * ux + vy * vy);
+    const double tna = 4.0 * (w0[3] / rho - kin) / (double)(ck + 2);
+    const double lama = 1.0 / tna;
+    const double T = th[3];
+    double red[10];
+#pragma unroll
+    for (int a = 0; a < 10; ++a) red[a] = 0.0;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
+        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+        double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+        mom_add(red + 0, V.w[r], V.u[r], V.v[r], coeff * cx * shh * mh, coeff * cx * shb * mb);
+        mom_add(red + 4, V.w[r], V.u[r], V.v[r], coeff * cy * shh * mh, coeff * cy * shb * mb);
+        double ca2 = (V.u[r] - ux) * (V.u[r] - ux) + (V.v[r] - vy) * (V.v[r] - vy);
+        double eah = rho * lama / UGKS_PI * exp(-lama * ca2);
+        double eab = 0.5 * (double)ck * tna * eah;
+        double dh = f0h[r] - eah, db = f0b[r] - eab;
+        red[8] += 0.5 * V.w[r] * cx * (c2 * dh + db);
+        red[9] += 0.5 * V.w[r] * cy * (c2 * dh + db);
+    }
+    block_sum<10>(red, lds);
+    double tdx[4], tdy[4], tnl[4], tnr[4], ttg[4];
+    if (!solve4(J, red + 0, tdx) || !solve4(J, red + 4, tdy) || !solve4(J, mnl, tnl) || !solve4(J, mnr, tnr) ||
+        !solve4(J, mtg, ttg))
+        return ST_SOLVE;
+    const double q0x = red[8], q0y = red[9];
+    // time slope moments
+    double wt4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        double qth[4], qtb[4];
+        qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
+        double nlh = 0, nlb = 0, nrh = 0, nrb = 0, tgh = 0, tgb = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            nlh += qth[a] * tnl[a]; nlb += qtb[a] * tnl[a];
+            nrh += qth[a] * tnr[a]; nrb += qtb[a] * tnr[a];
+            tgh += qth[a] * ttg[a]; tgb += qtb[a] * ttg[a];
+        }
+        double vn = axis == 1 ? V.u[r] : V.v[r], vt = axis == 1 ? V.v[r] : V.u[r];
+        double th_ = vn * (vn >= 0.0 ? nlh : nrh) + vt * tgh;
+        double tb_ = vn * (vn >= 0.0 ? nlb : nrb) + vt * tgb;
+        mom_add(wt4, V.w[r], V.u[r], V.v[r], th_, tb_);
+    }
+    block_sum<4>(wt4, lds);
+    double wtime[4] = {-wt4[0], -wt4[1], -wt4[2], -wt4[3]}, ttm[4];
+    if (!solve4(J, wtime, ttm)) return ST_SOLVE;
+    // variable-tau time integrals
+    double w0c[4] = {w0[0], w0[1], w0[2], w0[3]};
+    const double tau = tau_of_w(D.gas, w0c);
+    const double ex = exp(-dt / tau);
+    double mt[5];
+    mt[3] = tau * (1.0 - ex);
+    mt[4] = -tau * dt * ex + tau * mt[3];
+    mt[0] = dt - mt[3];
+    mt[1] = -tau * mt[0] + mt[4];
+    mt[2] = 0.5 * dt * dt - tau * mt[0];
+    double fw[4] = {0, 0, 0, 0};
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        double qth[4], qtb[4];
+        qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
+        double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
+        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+        double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+        double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
+        double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
+        double dxh = 0, dxb = 0, dyh = 0, dyb = 0, nlh = 0, nlb = 0, nrh = 0, nrb = 0, tgh = 0, tgb = 0, tmh = 0,
+               tmb = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            dxh += qth[a] * tdx[a]; dxb += qtb[a] * tdx[a];
+            dyh += qth[a] * tdy[a]; dyb += qtb[a] * tdy[a];
+            nlh += qth[a] * tnl[a]; nlb += qtb[a] * tnl[a];
+            nrh += qth[a] * tnr[a]; nrb += qtb[a] * tnr[a];
+            tgh += qth[a] * ttg[a]; tgb += qtb[a] * ttg[a];
+            tmh += qth[a] * ttm[a]; tmb += qtb[a] * ttm[a];
+        }
+        kxh -= dxh; kxb -= dxb; kyh -= dyh; kyb -= dyb;
+        double s0h = mh + kxh * q0x + kyh * q0y;
+        double s0b = mb + kxb * q0x + kyb * q0y;
+        double vn = axis == 1 ? V.u[r] : V.v[r], vt = axis == 1 ? V.v[r] : V.u[r];
+        double slh = vn * (vn >= 0.0 ? nlh : nrh) + vt * tgh;
+        double slb = vn * (vn >= 0.0 ? nlb : nrb) + vt * tgb;
+        double fh = vn * (mt[0] * s0h + mt[1] * slh + mt[2] * tmh + mt[3] * f0h[r] - mt[4] * (vn * fnh[r] + vt * fth[r]));
+        double fb = vn * (mt[0] * s0b + mt[1] * slb + mt[2] * tmb + mt[3] * f0b[r] - mt[4] * (vn * fnb[r] + vt * ftb[r]));
+        fin = fin && isfinite(fh) && isfinite(fb);
+        mom_add(fw, V.w[r], V.u[r], V.v[r], fh, fb);
+        acc_h[r] += gwa * fh;
+        acc_b[r] += gwa * fb;
+    }
+    block_sum<4>(fw, lds);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fw_acc[a] += gwa * fw[a];
+    if (!__syncthreads_and(fin ? 1 : 0)) return ST_NONFINITE;
+    return 0;
+}
+
+template <int KPT, int NT>
+__global__ void __launch_bounds__(NT) k_face_flux(Dev D) {
+    __shared__ double lds[(NT / 64) * 20];
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    const int f = blockIdx.x;
+    const int nfx = D.ncell;
+    int axis, cl, cr;
+    const double* rec;
+    if (f < nfx) {
+        axis = 1;
+        int j = f / nx, i = f % nx;
+        cl = f;
+        cr = j * nx + (i == nx - 1 ? 0 : i + 1);
+        rec = D.xf + (size_t)f * 18 * nvel;
+    } else {
+        axis = 2;
+        int g = f - nfx;   // face between rows j and j+1, g = j*nx+i
+        cl = g;
+        cr = g + nx;
+        rec = D.yf + (size_t)g * 18 * nvel;
+    }
+    VelR<KPT, NT> V;
+    load_vel<KPT, NT>(D, V);
+    const double dt = D.scal[SC_DT];
+    double acc_h[KPT], acc_b[KPT], fw_acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) { acc_h[r] = 0.0; acc_b[r] = 0.0; }
+    const double ln = axis == 1 ? D.dx : D.dy, lt = axis == 1 ? D.dy : D.dx;
+    const int sl = axis == 1 ? 1 : 3, sr = axis == 1 ? 0 : 2;
+    unsigned st = 0;
+    for (int a = 0; a < 3 && st == 0; ++a) {
+        double f0h[KPT], f0b[KPT], fnh[KPT], fnb[KPT], fth[KPT], ftb[KPT];
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            int k = threadIdx.x + NT * r;
+            const double* p = rec + (size_t)(a * 6) * nvel + k;
+            f0h[r] = p[0]; fnh[r] = p[(size_t)nvel]; fth[r] = p[2 * (size_t)nvel];
+            f0b[r] = p[3 * (size_t)nvel]; fnb[r] = p[4 * (size_t)nvel]; ftb[r] = p[5 * (size_t)nvel];
+        }
+        double mnl[4], mnr[4], mtg[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double* pl = D.ws + (size_t)(cl * 4 + m) * 24 + (sl * 3 + a) * 2;
+            const double* pr = D.ws + (size_t)(cr * 4 + m) * 24 + (sr * 3 + a) * 2;
+            mnl[m] = pl[0] / ln;
+            mnr[m] = pr[0] / ln;
+            mtg[m] = 0.5 * (pl[1] + pr[1]) / lt;
+        }
+        st = ugks_node<KPT, NT>(D, V, axis, dt, f0h, f0b, fnh, fnb, fth, ftb, mnl, mnr, mtg, KC.gw[a], acc_h, acc_b,
+                                fw_acc, lds);
+    }
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st);
+        return;
+    }
+    double *oh, *ob, *ow;
+    if (axis == 1) {
+        oh = D.xfl_h + (size_t)f * nvel; ob = D.xfl_b + (size_t)f * nvel; ow = D.xfw + 4 * (size_t)f;
+    } else {
+        size_t fo = (size_t)(f - nfx) + nx;   // south face of row j+1
+        oh = D.yfl_h + fo * nvel; ob = D.yfl_b + fo * nvel; ow = D.yfw + 4 * fo;
+    }
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        oh[k] = acc_h[r];
+        ob[k] = acc_b[r];
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 4; ++m) ow[m] = fw_acc[m];
+    (void)ny;
+}
+
+// ===========================================================================
+// multiscale diffuse wall (physical gas): one workgroup per wall face
+// ===========================================================================
+__device__ inline void micro_slope(int ck, const double prim[4], const double sw[4], double a[4]) {
+    a[3] = 4.0 * prim[3] * prim[3] / ((ck + 2) * prim[0]) *
+           (2.0 * sw[3] - 2.0 * prim[1] * sw[1] - 2.0 * prim[2] * sw[2] +
+            sw[0] * (prim[1] * prim[1] + prim[2] * prim[2] - 0.5 * (ck + 2) / prim[3]));
+    a[2] = 2.0 * prim[3] / prim[0] * (sw[2] - prim[2] * sw[0]) - prim[2] * a[3];
+    a[1] = 2.0 * prim[3] / prim[0] * (sw[1] - prim[1] * sw[0]) - prim[1] * a[3];
+    a[0] = sw[0] / prim[0] - prim[1] * a[1] - prim[2] * a[2] -
+           0.5 * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (ck + 2) / prim[3]) * a[3];
+}
+
+__device__ inline void half_moments(int ck, const double prim[4], double mu[7], double mv[6], double mxi[3]) {
+    double mul[7], mur[7];
+    mul[0] = 0.5 * erfc(-sqrt(prim[3]) * prim[1]);
+    mul[1] = prim[1] * mul[0] + 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
+    mur[0] = 0.5 * erfc(sqrt(prim[3]) * prim[1]);
+    mur[1] = prim[1] * mur[0] - 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
+    for (int i = 2; i <= 6; ++i) {
+        mul[i] = prim[1] * mul[i - 1] + 0.5 * (i - 1) * mul[i - 2] / prim[3];
+        mur[i] = prim[1] * mur[i - 1] + 0.5 * (i - 1) * mur[i - 2] / prim[3];
+    }
+    for (int i = 0; i <= 6; ++i) mu[i] = mul[i] + mur[i];
+    mv[0] = 1.0;
+    mv[1] = prim[2];
+    for (int i = 2; i <= 5; ++i) mv[i] = prim[2] * mv[i - 1] + 0.5 * (i - 1) * mv[i - 2] / prim[3];
+    mxi[0] = 1.0;
+    mxi[1] = 0.5 * ck / prim[3];
+    mxi[2] = ck * (ck + 2.0) / (4.0 * prim[3] * prim[3]);
+}
+
+__device__ inline void m_uvxi(const double mu[7], const double mv[6], const double mxi[3], int al, int be, int de,
+                              double o[4]) {
+    o[0] = mu[al] * mv[be] * mxi[de / 2];
+    o[1] = mu[al + 1] * mv[be] * mxi[de / 2];
+    o[2] = mu[al] * mv[be + 1] * mxi[de / 2];
+    o[3] = 0.5 * (mu[al + 2] * mv[be] * mxi[de / 2] + mu[al] * mv[be + 2] * mxi[de / 2] +
+                  mu[al] * mv[be] * mxi[(de + 2) / 2]);
+}
+
+__device__ inline void m_auvxi(const double a[4], const double mu[7], const double mv[6], const double mxi[3], int al,
+                               int be, double o[4]) {
+    double t0[4], t1[4], t2[4], t3[4], t4[4], t5[4];
+    m_uvxi(mu, mv, mxi, al, be, 0, t0);
+    m_uvxi(mu, mv, mxi, al + 1, be, 0, t1);
+    m_uvxi(mu, mv, mxi, al, be + 1, 0, t2);
+    m_uvxi(mu, mv, mxi, al + 2, be, 0, t3);
+    m_uvxi(mu, mv, mxi, al, be + 2, 0, t4);
+    m_uvxi(mu, mv, mxi, al, be, 2, t5);
+    for (int c = 0; c < 4; ++c)
+        o[c] = a[0] * t0[c] + a[1] * t1[c] + a[2] * t2[c] + 0.5 * a[3] * t3[c] + 0.5 * a[3] * t4[c] +
+               0.5 * a[3] * t5[c];
+}
+
+__device__ inline void wall_time_coeffs(double dt, double tau, double mt[5]) {
+    double x = dt / tau;
+    if (x < 0.1) {
+        double t = 1.0;
+        for (int a = 0; a < 5; ++a) mt[a] = 0.0;
+        for (int n = 1; n <= 24; ++n) {
+            t = t * (-x) / (double)n;
+            mt[3] = mt[3] - tau * t;
+            if (n >= 2) {
+                mt[0] = mt[0] + tau * t;
+                mt[4] = mt[4] + tau * tau * (double)(n - 1) * t;
+            }
+            if (n >= 3) {
+                mt[1] = mt[1] + tau * tau * (double)(n - 2) * t;
+                mt[2] = mt[2] - tau * tau * t;
+            }
+        }
+    } else {
+        double ex = exp(-x);
+        mt[3] = tau * (1.0 - ex);
+        mt[4] = -tau * dt * ex + tau * mt[3];
+        mt[0] = dt - mt[3];
+        mt[1] = -tau * mt[0] + mt[4];
+        mt[2] = 0.5 * dt * dt - tau * mt[0];
+    }
+}
+
+template <int KPT, int NT>
+__global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
+    __shared__ double lds[(NT / 64) * 20];
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
+    const int top = blockIdx.x >= nx ? 1 : 0, i = blockIdx.x % nx;
+    const int side = top ? 1 : -1;
+    const int c = top ? (ny - 1) * nx + i : i;
+    const double* rec = (top ? D.wt : D.wb) + (size_t)i * 18 * nvel;
+    VelR<KPT, NT> V;
+    load_vel<KPT, NT>(D, V);
+    const double dt = D.scal[SC_DT], dn = D.dy;
+    const double ux = D.wall_ux[top], theta = 0.5 * D.wall_tn[top];
+    const double* wc0 = D.W + 4 * c;
+    unsigned st = 0;
+    double p = (2.0 / (double)(ck + 2)) * (wc0[3] - 0.5 * (wc0[1] * wc0[1] + wc0[2] * wc0[2]) / wc0[0]);
+    if (!(p > 0.0) || !isfinite(p)) st = ST_WALL;
+    const double lambda = 0.5 / theta;
+    double prim_g[4] = {p / theta, 0.0, -ux, lambda};
+    double wc[4] = {wc0[0], wc0[2], -wc0[1], wc0[3]};
+    double wg[4] = {prim_g[0], 0.0, -prim_g[0] * ux, prim_g[0] * ((double)(ck + 2) / (4.0 * lambda) + 0.5 * ux * ux)};
+    double sw[4], an[4];
+    for (int a = 0; a < 4; ++a) sw[a] = (wg[a] - wc[a]) / ((double)side * 0.5 * dn);
+    micro_slope(ck, prim_g, sw, an);
+    double ww[4] = {prim_g[0], prim_g[0] * ux, 0.0, prim_g[0] * ((double)(ck + 2) * theta / 2.0 + ux * ux / 2.0)};
+    double th[4], J[16], mh[KPT];
+    if (st == 0 && !closure_solve<KPT, NT>(ck, ww, V, th, J, mh, lds)) st = ST_NEWTON;
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st);
+        return;
+    }
+    double mb[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) mb[r] = 0.5 * (double)ck * th[3] * mh[r];
+    double mu[7], mv[6], mxi[3];
+    half_moments(ck, prim_g, mu, mv, mxi);
+    double wtau[4] = {wg[0], -wg[2], wg[1], wg[3]};
+    const double tau_w = tau_of_w(D.gas, wtau);
+    double mt[5];
+    wall_time_coeffs(dt, tau_w, mt);
+    const double coeff = 0.8 * (1.0 - D.gas.pr) * lambda * lambda / prim_g[0];
+    double acc_h[KPT], acc_b[KPT], fw_acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) { acc_h[r] = 0.0; acc_b[r] = 0.0; }
+    for (int a = 0; a < 3 && st == 0; ++a) {
+        double hin[KPT], bin[KPT], hn[KPT], bn[KPT], ht[KPT], bt[KPT];
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            int k = threadIdx.x + NT * r;
+            const double* q = rec + (size_t)(a * 6) * nvel + k;
+            hin[r] = q[0]; hn[r] = q[(size_t)nvel]; ht[r] = q[2 * (size_t)nvel];
+            bin[r] = q[3 * (size_t)nvel]; bn[r] = q[4 * (size_t)nvel]; bt[r] = q[5 * (size_t)nvel];
+        }
+        double red[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            double vn = V.v[r], vt = -V.u[r], v2 = vn * vn + vt * vt;
+            red[0] += V.w[r] * (-ht[r]);
+            red[1] += V.w[r] * (-vn * ht[r]);
+            red[2] += V.w[r] * (-vt * ht[r]);
+            red[3] += V.w[r] * (-0.5 * (v2 * ht[r] + bt[r]));
+            double vtl = -V.u[r] + ux, v2l = vn * vn + vtl * vtl;
+            red[4] += 0.5 * V.w[r] * vn * (v2l * hin[r] + bin[r]);
+            red[5] += 0.5 * V.w[r] * vtl * (v2l * hin[r] + bin[r]);
+        }
+        block_sum<6>(red, lds);
+        double at[4], att[4], m1[4], m2[4], sw2[4];
+        micro_slope(ck, prim_g, red, at);
+        m_auvxi(an, mu, mv, mxi, 1, 0, m1);
+        m_auvxi(at, mu, mv, mxi, 0, 1, m2);
+        for (int q = 0; q < 4; ++q) sw2[q] = -prim_g[0] * (m1[q] + m2[q]);
+        micro_slope(ck, prim_g, sw2, att);
+        const bool shakhov = D.gas.pr != 1.0;
+        double ginh[KPT], ginb[KPT], ir[2] = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            double vn = V.v[r], vt = -V.u[r], v2 = vn * vn + vt * vt;
+            double corrh = 0.0, corrb = 0.0;
+            if (shakhov) {
+                double vtl = -V.u[r] + ux, v2l = vn * vn + vtl * vtl;
+                double qdot = vn * red[4] + vtl * red[5], shh = 2.0 * lambda * v2l - 4.0;
+                corrh = coeff * qdot * shh * mh[r];
+                corrb = coeff * qdot * (shh + 2.0 / (double)ck) * mb[r];
+            }
+            double Han = (an[0] + an[1] * vn + an[2] * vt) * mh[r] + 0.5 * an[3] * (v2 * mh[r] + mb[r]);
+            double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh[r] + 0.5 * at[3] * (v2 * mh[r] + mb[r]);
+            double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh[r] + 0.5 * att[3] * (v2 * mh[r] + mb[r]);
+            double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb[r] + 0.5 * an[3] * (v2 * mb[r] + mxi[2] * mh[r]);
+            double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb[r] + 0.5 * at[3] * (v2 * mb[r] + mxi[2] * mh[r]);
+            double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb[r] + 0.5 * att[3] * (v2 * mb[r] + mxi[2] * mh[r]);
+            ginh[r] = mt[0] * mh[r] + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin[r] -
+                      mt[4] * (vn * hn[r] + V.u[r] * ht[r]);
+            ginb[r] = mt[0] * mb[r] + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin[r] -
+                      mt[4] * (vn * bn[r] + V.u[r] * bt[r]);
+            ginh[r] = ginh[r] + mt[0] * corrh;
+            ginb[r] = ginb[r] + mt[0] * corrb;
+            if ((double)side * vn > 0.0)
+                ir[0] += V.w[r] * vn * ginh[r];
+            else
+                ir[1] += dt * V.w[r] * vn * (mh[r] / prim_g[0]);
+        }
+        block_sum<2>(ir, lds);
+        double inc = ir[0], refl = ir[1];
+        if (!isfinite(refl) || !(fabs(refl) > UGKS_DBL_MIN)) { st = ST_WALL; break; }
+        double rhow = -inc / refl;
+        if (!isfinite(rhow) || !(rhow > 0.0)) { st = ST_WALL; break; }
+        double fw[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            double fh, fb;
+            if ((double)side * V.v[r] > 0.0) {
+                fh = V.v[r] * ginh[r];
+                fb = V.v[r] * ginb[r];
+            } else {
+                fh = dt * V.v[r] * rhow * (mh[r] / prim_g[0]);
+                fb = dt * V.v[r] * rhow * (mb[r] / prim_g[0]);
+            }
+            mom_add(fw, V.w[r], V.u[r], V.v[r], fh, fb);
+            acc_h[r] += KC.gw[a] * fh;
+            acc_b[r] += KC.gw[a] * fb;
+        }
+        block_sum<4>(fw, lds);
+        if (!(fabs(fw[0]) <= UGKS_GATE * fmax(1.0, fabs(inc)))) { st = ST_WALL; break; }
+        for (int q = 0; q < 4; ++q) fw_acc[q] += KC.gw[a] * fw[q];
+    }
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st);
+        return;
+    }
+    size_t fo = (size_t)(top ? ny : 0) * nx + i;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        D.yfl_h[fo * nvel + k] = acc_h[r];
+        D.yfl_b[fo * nvel + k] = acc_b[r];
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) D.yfw[4 * fo + q] = fw_acc[q];
+}
+
+// ===========================================================================
+// cell kernel: volume targets (9 closures), physical collision, streaming
+// rates and the per-cell residual partial sums
+// ===========================================================================
+template <int KPT, int NT>
+__global__ void __launch_bounds__(NT) k_cell(Dev D) {
+    __shared__ double lds[(NT / 64) * 20];
+    const int nx = D.nx, nvel = D.nvel, ck = D.gas.ck;
+    const int c = blockIdx.x, i = c % nx, j = c / nx;
+    VelR<KPT, NT> V;
+    load_vel<KPT, NT>(D, V);
+    const double* Wc = D.W + 4 * c;
+    double W[4] = {Wc[0], Wc[1], Wc[2], Wc[3]};
+    // volume nodes of W and their Gauss mean
+    double nodes[4][9], avg[4] = {0, 0, 0, 0};
+    for (int m = 0; m < 4; ++m)
+        for (int q = 0; q < 9; ++q) nodes[m][q] = D.wv[(size_t)(c * 4 + m) * 9 + q];
+    for (int cc = 0; cc < 3; ++cc)
+        for (int a = 0; a < 3; ++a)
+            for (int m = 0; m < 4; ++m) avg[m] += KC.gw[a] * KC.gw[cc] * nodes[m][a * 3 + cc];
+    double amh[KPT], amb[KPT], akxh[KPT], akxb[KPT], akyh[KPT], akyb[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) amh[r] = amb[r] = akxh[r] = akxb[r] = akyh[r] = akyb[r] = 0.0;
+    unsigned st = 0;
+    for (int cc = 0; cc < 3 && st == 0; ++cc)
+        for (int a = 0; a < 3 && st == 0; ++a) {
+            double wq[4];
+            for (int m = 0; m < 4; ++m) wq[m] = nodes[m][a * 3 + cc] + W[m] - avg[m];
+            const double wt2 = KC.gw[a] * KC.gw[cc];
+            double th[4], J[16], qh[KPT];
+            if (!closure_solve<KPT, NT>(ck, wq, V, th, J, qh, lds)) { st = ST_NEWTON; break; }
+            double rho, ux, vy, thg;
+            if (!s2_prim(ck, wq, &rho, &ux, &vy, &thg)) { st = ST_NONFINITE; break; }
+            const double lam = 0.5 / thg, coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho, T = th[3];
+            double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < KPT; ++r) {
+                double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
+                double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+                double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+                mom_add(red + 0, V.w[r], V.u[r], V.v[r], coeff * cx * shh * mh, coeff * cx * shb * mb);
+                mom_add(red + 4, V.w[r], V.u[r], V.v[r], coeff * cy * shh * mh, coeff * cy * shb * mb);
+            }
+            block_sum<8>(red, lds);
+            double tdx[4], tdy[4];
+            if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) { st = ST_SOLVE; break; }
+#pragma unroll
+            for (int r = 0; r < KPT; ++r) {
+                double qth[4], qtb[4];
+                qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
+                double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
+                double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+                double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+                double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
+                double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
+                double dxh = 0, dxb = 0, dyh = 0, dyb = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    dxh += qth[q] * tdx[q]; dxb += qtb[q] * tdx[q];
+                    dyh += qth[q] * tdy[q]; dyb += qtb[q] * tdy[q];
+                }
+                amh[r] += wt2 * mh; amb[r] += wt2 * mb;
+                akxh[r] += wt2 * (kxh - dxh); akxb[r] += wt2 * (kxb - dxb);
+                akyh[r] += wt2 * (kyh - dyh); akyb[r] += wt2 * (kyb - dyb);
+            }
+        }
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st);
+        return;
+    }
+    // physical Shakhov collision of the current state
+    double hv[KPT], bv[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        hv[r] = D.h[(size_t)c * nvel + k];
+        bv[r] = D.b[(size_t)c * nvel + k];
+    }
+    double rho, ux, vy, thg;
+    if (!s2_prim(ck, W, &rho, &ux, &vy, &thg)) {
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)ST_NONFINITE);
+        return;
+    }
+    double q[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+        double dh = hv[r] - amh[r], db = bv[r] - amb[r];
+        q[0] += 0.5 * V.w[r] * cx * (c2 * dh + db);
+        q[1] += 0.5 * V.w[r] * cy * (c2 * dh + db);
+    }
+    block_sum<2>(q, lds);
+    const double tau = tau_of_w(D.gas, W);
+    const double dt = D.scal[SC_DT], inv_dt = 1.0 / dt;
+    const double area = D.dx * D.dy, ainv = 1.0 / area;
+    const int cw = j * nx + (i == 0 ? nx - 1 : i - 1);
+    const size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        double eh = amh[r] + akxh[r] * q[0] + akyh[r] * q[1];
+        double eb = amb[r] + akxb[r] * q[0] + akyb[r] * q[1];
+        double ch = (eh - hv[r]) / tau, cb = (eb - bv[r]) / tau;
+        double sh = (D.xfl_h[(size_t)cw * nvel + k] * D.dy - D.xfl_h[(size_t)c * nvel + k] * D.dy +
+                     D.yfl_h[fs * nvel + k] * D.dx - D.yfl_h[fn * nvel + k] * D.dx) * ainv * inv_dt;
+        double sb = (D.xfl_b[(size_t)cw * nvel + k] * D.dy - D.xfl_b[(size_t)c * nvel + k] * D.dy +
+                     D.yfl_b[fs * nvel + k] * D.dx - D.yfl_b[fn * nvel + k] * D.dx) * ainv * inv_dt;
+        double th_ = sh + ch, tb_ = sb + cb;
+        D.rh[(size_t)c * nvel + k] = th_;
+        D.rb[(size_t)c * nvel + k] = tb_;
+        fin = fin && isfinite(th_) && isfinite(tb_);
+        red[0] += V.w[r] * th_ * th_;
+        red[1] += V.w[r] * tb_ * tb_;
+        red[2] += V.w[r] * fmax(hv[r] * hv[r], eh * eh);
+        red[3] += V.w[r] * fmax(bv[r] * bv[r], eb * eb);
+        mom_add(red + 4, V.w[r], V.u[r], V.v[r], ch, cb);
+    }
+    block_sum<8>(red, lds);
+    if (!__syncthreads_and(fin ? 1 : 0)) st |= ST_NONFINITE;
+    if (threadIdx.x == 0) {
+        double rw[4], cm = 0.0, wm = 1.0;
+        for (int m = 0; m < 4; ++m) {
+            double fc = (D.xfw[4 * (size_t)cw + m] * D.dy - D.xfw[4 * (size_t)c + m] * D.dy +
+                         D.yfw[4 * fs + m] * D.dx - D.yfw[4 * fn + m] * D.dx) / area;
+            rw[m] = fc * inv_dt;
+            D.rate_w[4 * (size_t)c + m] = rw[m];
+            cm = fmax(cm, fabs(red[4 + m]));
+            wm = fmax(wm, fabs(W[m]));
+        }
+        if (tau * cm / wm > UGKS_GATE) st |= ST_GATE;
+        double* cp = D.cellpart + (size_t)c * 16;
+        cp[0] = area * red[0]; cp[1] = area * red[1]; cp[2] = area * red[2]; cp[3] = area * red[3];
+        for (int m = 0; m < 4; ++m) {
+            double t = area * rw[m];
+            cp[4 + m] = t * t;
+            cp[8 + m] = area * fabs(W[m] + dt * rw[m]);
+        }
+        cp[12] = area * area;
+        cp[13] = rw[0] * area;
+        if (st) atomicOr(D.status, st);
+    }
+}
+
+// ===========================================================================
+// residual diagnostics (one workgroup): sums of the per-cell partials
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_norms(Dev D, double residual_tol) {
+    __shared__ double lds[4 * 14];
+    double s[14];
+    for (int q = 0; q < 14; ++q) s[q] = 0.0;
+    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x)
+        for (int q = 0; q < 14; ++q) s[q] += D.cellpart[(size_t)c * 16 + q];
+    block_sum<14>(s, lds);
+    if (threadIdx.x != 0) return;
+    const double SMV = 1.17549435082228750797e-38;
+    double* o = D.scal + SC_NORMS;
+    const double N = (double)D.ncell;
+    double ordinary = fmax(fmax(s[8], s[11]), SMV);
+    for (int m = 0; m < 4; ++m) {
+        o[4 + m] = sqrt(N * s[4 + m]) / (s[8 + m] + SMV);
+        o[8 + m] = sqrt(s[4 + m] / fmax(s[12], SMV));
+        o[m] = (s[8 + m] <= sqrt(UGKS_DBL_EPS) * ordinary) ? o[8 + m] : o[4 + m];
+    }
+    double hr = sqrt(s[0] / fmax(s[2], SMV)), br = sqrt(s[1] / fmax(s[3], SMV));
+    if (D.scal[SC_REF_INIT] == 0.0) {
+        D.scal[SC_REF_H] = hr;
+        D.scal[SC_REF_B] = br;
+        D.scal[SC_REF_INIT] = 1.0;
+    }
+    double rh = D.scal[SC_REF_H], rbv = D.scal[SC_REF_B];
+    double nz = fmax(100.0 * residual_tol, 1000.0 * UGKS_DBL_EPS);
+    o[14] = rh > UGKS_DBL_MIN ? hr / rh : hr;
+    o[15] = rbv > UGKS_DBL_MIN ? br / rbv : br;
+    o[16] = hr;
+    o[17] = br;
+    o[12] = rh <= nz ? hr : hr / rh;
+    o[13] = rbv <= nz ? br : br / rbv;
+    o[18] = 0.0;
+    o[19] = fabs(s[13]);
+    o[20] = sqrt(s[0]);
+    o[21] = sqrt(s[1]);
+    o[22] = rh;
+    o[23] = rbv;
+    // wall flux sums of the evaluation (PhysicalDiagnostic layout: left, right, bottom, top)
+    double* wl = D.scal + SC_WALLS;
+    const double dt = D.scal[SC_DT];
+    for (int q = 0; q < 16; ++q) wl[q] = 0.0;
+    for (int jj = 0; jj < D.ny; ++jj)
+        for (int m = 0; m < 4; ++m) {
+            double f = D.xfw[4 * (size_t)(jj * D.nx + D.nx - 1) + m] * D.dy / dt;
+            wl[0 * 4 + m] -= f;
+            wl[1 * 4 + m] += f;
+        }
+    for (int ii = 0; ii < D.nx; ++ii)
+        for (int m = 0; m < 4; ++m) {
+            wl[2 * 4 + m] -= D.yfw[4 * (size_t)ii + m] * D.dx / dt;
+            wl[3 * 4 + m] += D.yfw[4 * (size_t)(D.ny * D.nx + ii) + m] * D.dx / dt;
+        }
+}
+
+// ===========================================================================
+// cold start: every cell holds the discrete conservative equilibrium of W0
+// ===========================================================================
+template <int KPT, int NT>
+__global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux, double tn) {
+    __shared__ double lds[(NT / 64) * 20];
+    VelR<KPT, NT> V;
+    load_vel<KPT, NT>(D, V);
+    const double lam = 1.0 / tn;
+    double w[4] = {rho, rho * ux, 0.0, 0.5 * rho / (lam * (D.gas.gamma - 1.0)) + 0.5 * rho * (ux * ux)};
+    double th[4], J[16], qh[KPT];
+    if (!closure_solve<KPT, NT>(D.gas.ck, w, V, th, J, qh, lds)) {
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)ST_NEWTON);
+        return;
+    }
+    for (int c = blockIdx.x; c < D.ncell; c += gridDim.x) {
+#pragma unroll
+        for (int r = 0; r < KPT; ++r) {
+            int k = threadIdx.x + NT * r;
+            D.h[(size_t)c * D.nvel + k] = qh[r];
+            D.b[(size_t)c * D.nvel + k] = 0.5 * (double)D.gas.ck * th[3] * qh[r];
+        }
+        if (threadIdx.x < 4) D.W[4 * c + threadIdx.x] = w[threadIdx.x];
+    }
+}
+
+// ===========================================================================
+// observables of the current state: rho, U, V, T, p, stresses, heat flux
+// ===========================================================================
+template <int KPT, int NT>
+__global__ void __launch_bounds__(NT) k_observables(Dev D, double* out) {
+    __shared__ double lds[(NT / 64) * 20];
+    const int c = blockIdx.x, nvel = D.nvel;
+    double prim[4];
+    primary(D.gas.gamma, D.W + 4 * c, prim);
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        int k = threadIdx.x + NT * r;
+        double h = D.h[(size_t)c * nvel + k], b = D.b[(size_t)c * nvel + k], w = D.w[k];
+        double dvn = D.u[k] - prim[1], dvt = D.v[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
+        s[0] += w * v2 * h;
+        s[1] += w * b;
+        s[2] += w * dvn * dvn * h;
+        s[3] += w * dvn * dvt * h;
+        s[4] += w * dvt * dvt * h;
+        s[5] += 0.5 * w * dvn * (v2 * h + b);
+        s[6] += 0.5 * w * dvt * (v2 * h + b);
+    }
+    block_sum<7>(s, lds);
+    if (threadIdx.x == 0) {
+        double pr = (s[0] + s[1]) / (D.gas.ck + 2.0);
+        double* o = out + 10 * (size_t)c;
+        o[0] = prim[0]; o[1] = prim[1]; o[2] = prim[2]; o[3] = 1.0 / prim[3];
+        o[4] = 0.5 * prim[0] / prim[3];
+        o[5] = s[2] - pr; o[6] = s[3]; o[7] = s[4] - pr; o[8] = s[5]; o[9] = s[6];
+    }
+}
+
+extern "C" hipError_t ugks_launch_cold_start(const Dev* Dp, double rho, double ux, double tn, hipStream_t s) {
+    const Dev& D = *Dp;
+    int nb = D.ncell < 1024 ? D.ncell : 1024;
+    switch (D.nvel) {
+    case 256: hipLaunchKernelGGL((k_cold_start<4, 64>), dim3(nb), dim3(64), 0, s, D, rho, ux, tn); break;
+    case 1024: hipLaunchKernelGGL((k_cold_start<4, 256>), dim3(nb), dim3(256), 0, s, D, rho, ux, tn); break;
+    case 4096: hipLaunchKernelGGL((k_cold_start<8, 512>), dim3(nb), dim3(512), 0, s, D, rho, ux, tn); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ugks_launch_observables(const Dev* Dp, double* out, hipStream_t s) {
+    const Dev& D = *Dp;
+    switch (D.nvel) {
+    case 256: hipLaunchKernelGGL((k_observables<4, 64>), dim3(D.ncell), dim3(64), 0, s, D, out); break;
+    case 1024: hipLaunchKernelGGL((k_observables<4, 256>), dim3(D.ncell), dim3(256), 0, s, D, out); break;
+    case 4096: hipLaunchKernelGGL((k_observables<8, 512>), dim3(D.ncell), dim3(512), 0, s, D, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+extern "C" hipError_t ugks_set_consts(const Consts* k) { return hipMemcpyToSymbol(HIP_SYMBOL(KC), k, sizeof(Consts)); }
+
+template <int KPT, int NT>
+static void launch_closure_kernels(const Dev& D, hipStream_t s) {
+    hipLaunchKernelGGL((k_face_flux<KPT, NT>), dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL((k_wall_flux<KPT, NT>), dim3(2 * D.nx), dim3(NT), 0, s, D);
+}
+
+template <int KPT, int NT>
+static void launch_cell_kernel(const Dev& D, hipStream_t s) {
+    hipLaunchKernelGGL((k_cell<KPT, NT>), dim3(D.ncell), dim3(NT), 0, s, D);
+}
+
+extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vmax, double residual_tol,
+                                           hipStream_t s) {
+    const Dev& D = *Dp;
+    hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax);
+    hipLaunchKernelGGL(k_wrecon, dim3((4 * D.ncell + 255) / 256), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_recon, dim3((D.nvel + 127) / 128, D.ncell), dim3(128), 0, s, D);
+    switch (D.nvel) {
+    case 256: launch_closure_kernels<4, 64>(D, s); launch_cell_kernel<4, 64>(D, s); break;
+    case 1024: launch_closure_kernels<4, 256>(D, s); launch_cell_kernel<4, 256>(D, s); break;
+    case 4096: launch_closure_kernels<8, 512>(D, s); launch_cell_kernel<8, 512>(D, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, s, D, residual_tol);
+    return hipGetLastError();
+}

@@ -1,0 +1,56 @@
+// Device-resident state of one UGKS2D Couette instance and the argument
+// block every kernel receives by value.
+#pragma once
+#include "ugks_device.h"
+
+// index of the device scalars block (Dev::scal)
+enum ScalarSlot {
+    SC_DT = 0,          // physical dt (global CFL)
+    SC_DTP,             // pseudo time step dt*nCFL/CFL
+    SC_REF_H, SC_REF_B, // residual normalisation epoch (raw h/b norms of the first evaluation)
+    SC_REF_INIT,        // 1 once the epoch is set
+    SC_M0,              // closed-Couette mass target
+    SC_M0_INIT,
+    SC_MASS_BEFORE,     // sum A*rho of the proposal
+    SC_SCALE,           // mass scale M0/mass_before
+    SC_MASS_AFTER,
+    SC_REFL_BOT, SC_REFL_TOP,   // reflection normalisers of the frozen wall closure
+    SC_NORMS = 16,      // 24 residual diagnostics (layout of ugks_norms)
+    SC_WALLS = 40,      // 16 wall fluxes of the last evaluation
+    SC_COUNT = 64
+};
+
+struct Dev {
+    int nx, ny, nu, nv, nvel, ncell;
+    double dx, dy, cfl, ncfl;
+    Gas gas;
+    double wall_ux[2], wall_tn[2];
+    int macro_sweeps;
+    int tile;             // velocities per sweep workgroup
+    int ntiles;
+    // velocity table
+    const double *u, *v, *w;
+    // state and proposal
+    double *W, *h, *b;
+    double *Wn, *hn, *bn;
+    // reconstruction outputs (upwind face data, [face][node][6][nvel])
+    double *xf, *yf, *wb, *wt;
+    // macro reconstruction: ws[cell][m][side][node][2] (dn, dt per cell width), wv[cell][m][9]
+    double *ws, *wv;
+    // face fluxes (carry dt, per unit length): x: east face of cell, y: south face of row j (ny+1 rows)
+    double *xfl_h, *xfl_b, *yfl_h, *yfl_b, *xfw, *yfw;
+    // rates: rt = stream + physical collision, later the LU-SGS right-hand side
+    double *rh, *rb;
+    double *rate_w;       // [ncell][4]
+    double *cellpart;     // [ncell][16] per-cell residual partial sums
+    // implicit
+    double *dW, *dWs;     // [ncell][4] macro increment and its last-exchange snapshot
+    double *gx, *gy;      // spectral radii: east face of cell, south face of row j
+    double *w_tilde;      // [ncell][4]
+    double *tau_t;        // [ncell]
+    double *dh, *db;      // forward-sweep micro increments [ncell][nvel]
+    double *wallpart;     // [ntiles][2][nx] incidence partials of the wall rows
+    double *mompart;      // [ntiles][ncell][4] moment partials of the proposal
+    double *scal;         // ScalarSlot
+    unsigned *status;
+};
