@@ -105,6 +105,11 @@ int ugks_get_state_device(void* handle, double* W, double* h, double* b);
 /* Synchronize the instance's stream. */
 int ugks_sync(void* handle);
 
+/* Diagnostics (no counterpart in the reference): copy internal buffer
+ * `which` (0 x-face upwind data, 1 y-face data, 2/3 bottom/top wall data,
+ * 4 macro face slopes, 5 macro volume nodes, 6 device scalars) to out. */
+int ugks_debug_fetch(void* handle, int which, double* out, long n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,161 @@
+// CPU thread emulation of the HIP subset used by the UGKS2D kernels.
+//
+// TEST INFRASTRUCTURE ONLY.  tests/emu/build_emu.py compiles the library's
+// own .hip sources with g++ against this header into tests/emu/libugks_emu.so
+// so that kernel logic (indexing, layouts, reductions, wavefront order) can be
+// checked against the oracle on a machine without a GPU.  Each workgroup runs
+// as blockDim OS threads; barriers, lane shuffles and LDS are emulated per
+// workgroup, workgroups run one after another.  The product library
+// libugks_hip.so never includes this header.
+#pragma once
+#define UGKS_EMULATE 1
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using std::isfinite;
+
+#define __global__
+#define __device__
+#define __host__
+#define __constant__
+#define __shared__ static
+#define __launch_bounds__(...)
+#define __forceinline__ inline
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx { unsigned x, y, z; };
+
+struct EmuBlock {
+    explicit EmuBlock(int n) : bar(n) {}
+    std::barrier<> bar;
+    std::atomic<int> fails{0};
+    double xbuf[1024];
+    double rbuf[1024 * 24];
+};
+
+inline thread_local emu_idx threadIdx, blockIdx;
+inline thread_local emu_idx blockDim, gridDim;
+inline thread_local EmuBlock* emu_block = nullptr;
+inline thread_local char* emu_dyn = nullptr;
+
+#define UGKS_DYN_SHARED(T, name) T* name = (T*)emu_dyn
+
+inline void __syncthreads() { emu_block->bar.arrive_and_wait(); }
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline int __syncthreads_and(int p) {
+    EmuBlock& B = *emu_block;
+    B.bar.arrive_and_wait();
+    if (!p) B.fails.fetch_add(1);
+    B.bar.arrive_and_wait();
+    int r = B.fails.load() == 0;
+    B.bar.arrive_and_wait();
+    if (threadIdx.x == 0) B.fails.store(0);
+    B.bar.arrive_and_wait();
+    return r;
+}
+inline double __shfl_xor(double x, int mask, int width = 64) {
+    EmuBlock& B = *emu_block;
+    B.bar.arrive_and_wait();
+    B.xbuf[threadIdx.x] = x;
+    B.bar.arrive_and_wait();
+    unsigned lane = threadIdx.x % width, base = threadIdx.x - lane;
+    return B.xbuf[base + (lane ^ (unsigned)mask)];
+}
+inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+
+// ---- runtime API --------------------------------------------------------
+typedef int hipError_t;
+typedef void* hipStream_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorMemoryAllocation = 2 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+inline const char* hipGetErrorString(hipError_t e) { return e ? "emulation error" : "no error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+// Device allocations and dynamic LDS start as NaN (all bits set), so a read
+// of memory no kernel has written shows up in the results.
+inline hipError_t hipMalloc(void** p, size_t n) {
+    *p = std::malloc(n ? n : 1);
+    if (!*p) return hipErrorMemoryAllocation;
+    std::memset(*p, 0xff, n);
+    return hipSuccess;
+}
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemset(void* p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+#define HIP_SYMBOL(x) (x)
+template <typename T>
+inline hipError_t hipMemcpyToSymbol(T& sym, const void* src, size_t n) { std::memcpy((void*)&sym, src, n); return hipSuccess; }
+
+template <typename F, typename... A>
+inline void emu_launch(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args) {
+    std::vector<char> dyn(shmem + 64, (char)0xff);
+    const int nt = (int)(b.x * b.y * b.z);
+    const long nb = (long)g.x * g.y * g.z;
+    std::barrier<> outer(nt);
+    EmuBlock* cur = nullptr;
+    std::vector<std::thread> ts;
+    ts.reserve(nt);
+    for (int t = 0; t < nt; ++t)
+        ts.emplace_back([&, t]() {
+            for (long q = 0; q < nb; ++q) {
+                if (t == 0) { delete cur; cur = new EmuBlock(nt); }
+                outer.arrive_and_wait();
+                unsigned bx = q % g.x, by = (q / g.x) % g.y, bz = q / ((long)g.x * g.y);
+                threadIdx = {(unsigned)t % b.x, ((unsigned)t / b.x) % b.y, (unsigned)t / (b.x * b.y)};
+                blockIdx = {bx, by, bz};
+                blockDim = {b.x, b.y, b.z};
+                gridDim = {g.x, g.y, g.z};
+                emu_block = cur;
+                emu_dyn = dyn.data();
+                f(args...);
+                cur->bar.arrive_and_drop();
+                outer.arrive_and_wait();
+            }
+        });
+    for (auto& th : ts) th.join();
+    delete cur;
+}
+
+// workgroup reductions: one exchange instead of lane butterflies
+#define UGKS_EMU_REDUCTIONS 1
+template <int R>
+inline void block_sum(double* v, double*) {
+    EmuBlock& B = *emu_block;
+    B.bar.arrive_and_wait();
+    for (int r = 0; r < R; ++r) B.rbuf[threadIdx.x * R + r] = v[r];
+    B.bar.arrive_and_wait();
+    const int n = (int)blockDim.x;
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        for (int t = 0; t < n; ++t) s += B.rbuf[t * R + r];
+        v[r] = s;
+    }
+}
+template <int R>
+inline void block_max(double* v, double*) {
+    EmuBlock& B = *emu_block;
+    B.bar.arrive_and_wait();
+    for (int r = 0; r < R; ++r) B.rbuf[threadIdx.x * R + r] = v[r];
+    B.bar.arrive_and_wait();
+    const int n = (int)blockDim.x;
+    for (int r = 0; r < R; ++r) {
+        double s = B.rbuf[r];
+        for (int t = 1; t < n; ++t) s = std::fmax(s, B.rbuf[t * R + r]);
+        v[r] = s;
+    }
+}
+#define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), sh, st, __VA_ARGS__)

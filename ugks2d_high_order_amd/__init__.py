@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libugks_hip.so")
 EXPORTS = [
     "ugks_create", "ugks_destroy", "ugks_last_error", "ugks_set_state", "ugks_get_state", "ugks_cold_start",
     "ugks_evolution", "ugks_norms", "ugks_implicit_step", "ugks_run", "ugks_observables",
-    "ugks_set_state_device", "ugks_get_state_device", "ugks_sync",
+    "ugks_set_state_device", "ugks_get_state_device", "ugks_sync", "ugks_debug_fetch",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH):
     L.ugks_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.ugks_observables.argtypes = [C.c_void_p, _D]
     L.ugks_sync.argtypes = [C.c_void_p]
+    L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
     _lib = L
     return L
 
@@ -185,6 +186,11 @@ class UGKS:
         f = np.zeros((self.ncell, 10))
         _check(load_library().ugks_observables(self.handle, _ptr(f)))
         return f
+
+    def debug_fetch(self, which: int, n: int):
+        out = np.zeros(n)
+        _check(load_library().ugks_debug_fetch(self.handle, int(which), _ptr(out), int(n)))
+        return out
 
     def sync(self):
         _check(load_library().ugks_sync(self.handle))

@@ -3,7 +3,9 @@
 // pseudo-time iteration, and the fail-stop checks of the reference's gates.
 #include "../../include/ugks_hip.h"
 
+#ifndef UGKS_EMULATE
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cmath>
 #include <cstdio>
@@ -120,8 +122,9 @@ int status_to_code(unsigned st) {
 
 std::string status_text(unsigned st) {
     std::string s;
-    const char* names[] = {"newton", "gate", "nonfinite", "wall", "macro_alpha", "micro_alpha", "projection", "solve"};
-    for (int b = 0; b < 8; ++b)
+    const char* names[] = {"newton", "gate", "nonfinite", "wall", "macro_alpha", "micro_alpha", "projection", "solve",
+                           "in_face_flux", "in_wall_flux", "in_cell", "in_predict", "in_sweep", "in_macro", "in_cold_start"};
+    for (int b = 0; b < 15; ++b)
         if (st & (1u << b)) s += std::string(s.empty() ? "" : ",") + names[b];
     return s;
 }
@@ -364,6 +367,24 @@ extern "C" int ugks_run(void* handle, int n, int* done) {
         if (!acc) break;
         ++*done;
     }
+    return E_OK;
+}
+
+// Diagnostics: copy an internal buffer to the host.  which: 0 xf (upwind x
+// face data), 1 yf, 2 wb, 3 wt, 4 ws (macro face slopes), 5 wv (macro volume
+// nodes), 6 scalars.  n = number of doubles copied (clamped to the buffer).
+extern "C" int ugks_debug_fetch(void* handle, int which, double* out, long n) {
+    Instance* I = (Instance*)handle;
+    if (!I || !out) return fail(E_ARG, "null argument");
+    const Dev& D = I->D;
+    const size_t nv = D.nvel, nc = D.ncell;
+    const double* src[] = {D.xf, D.yf, D.wb, D.wt, D.ws, D.wv, D.scal};
+    const size_t len[] = {nc * 18 * nv, (size_t)D.nx * (D.ny - 1) * 18 * nv, (size_t)D.nx * 18 * nv,
+                          (size_t)D.nx * 18 * nv, nc * 96, nc * 36, SC_COUNT};
+    if (which < 0 || which > 6) return fail(E_ARG, "unknown buffer");
+    size_t m = std::min((size_t)n, len[which]);
+    HIPCHK(hipMemcpyAsync(out, src[which], sizeof(double) * m, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
     return E_OK;
 }
 

@@ -7,8 +7,15 @@
 // lanes, and the per-wave partials are added in wave order from LDS.
 #pragma once
 
+#ifndef UGKS_EMULATE
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
+
+// dynamically sized LDS array of a kernel
+#ifndef UGKS_DYN_SHARED
+#define UGKS_DYN_SHARED(T, name) extern __shared__ T name[]
+#endif
 
 #define UGKS_PI 3.14159265358979323846264338327950288
 #define UGKS_GATE 2.0e-12
@@ -26,6 +33,9 @@ enum : unsigned {
     ST_MICRO_ALPHA = 32u,  // micro candidate not admissible at alpha = 1
     ST_PROJECT = 64u,      // moment projection failed
     ST_SOLVE = 128u,       // singular 4x4 system
+    // where the failure happened
+    ST_AT_FACE = 1u << 8, ST_AT_WALL = 1u << 9, ST_AT_CELL = 1u << 10, ST_AT_PREDICT = 1u << 11,
+    ST_AT_SWEEP = 1u << 12, ST_AT_MACRO = 1u << 13, ST_AT_COLD = 1u << 14,
 };
 
 struct Consts {
@@ -49,6 +59,7 @@ __device__ inline double wave_max(double x) {
     return x;
 }
 
+#ifndef UGKS_EMU_REDUCTIONS
 // Sum R values over the whole workgroup; every thread gets the totals.
 // lds must hold (blockDim.x/64)*R doubles.
 template <int R>
@@ -88,6 +99,8 @@ __device__ inline void block_max(double* v, double* lds) {
         v[r] = s;
     }
 }
+
+#endif
 
 // moments (rho, rho U, rho V, rho E) contribution of one velocity
 __device__ inline void mom_add(double m[4], double w, double u, double v, double h, double b) {

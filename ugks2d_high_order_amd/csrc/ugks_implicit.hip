@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) k_macro_pr2(Dev D) {
     for (int sweep = 0; sweep < D.macro_sweeps; ++sweep) {
         for (int dd = 0; dd < 2 * nd; ++dd) {
             int d = dd < nd ? dd : 2 * nd - 1 - dd;
-            int ilo = max(0, d - (ny - 1)), ihi = min(nx - 1, d);
+            int ilo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, ihi = d < nx - 1 ? d : nx - 1;
             for (int i = ilo + (int)threadIdx.x; i <= ihi; i += blockDim.x) ok &= relax_macro(D, i, d - i, dtp) ? 1 : 0;
             __threadfence_block();
             __syncthreads();
@@ -324,7 +324,7 @@ struct MicroCoef {
 // x seam and on the old side are zero (reset), the wall closure of the
 // forward sweep is frozen from a zero increment
 __global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
-    extern __shared__ double sm[];   // [2][2][nx][T]
+    UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel;
     const int lane = threadIdx.x % T, slot = threadIdx.x / T, nslot = blockDim.x / T;
     const int k = blockIdx.x * T + lane;
@@ -337,9 +337,15 @@ __global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
     for (int d = 0; d < nd; ++d) {
         double* cur = buf[d & 1];
         const double* prv = buf[(d + 1) & 1];
-        const int ilo = max(0, d - (ny - 1)), ihi = min(nx - 1, d);
-        for (int i = ilo + slot; i <= ihi; i += nslot) {
+        const int ilo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, ihi = d < nx - 1 ? d : nx - 1;
+        // every thread runs the same trip count so the lane-group
+        // reductions below execute with converged workgroups
+        for (int base = ilo; base <= ihi; base += nslot) {
+            const int i = base + slot;
+            const bool act = i <= ihi;
             const int j = d - i, c = j * nx + i;
+            double part = 0.0;
+            if (act) {
             const size_t o = (size_t)c * nvel + k;
             const double tt = D.tau_t[c];
             double rh = D.rh[o], rb = D.rb[o];
@@ -368,11 +374,15 @@ __global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
             D.db[o] = db;
             // incidence partial of the wall closure used by the backward sweep
             if (j == 0 || j == ny - 1) {
-                const int top = j == ny - 1 ? 1 : 0;
-                const double vn = top ? vk : -vk;
-                double part = (vn < 0.0) ? 0.0 : wk * vn * dh;
-                for (int m = T >> 1; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
-                if (lane == 0) D.wallpart[((size_t)blockIdx.x * 2 + top) * nx + i] = part;
+                const double vn = (j == ny - 1) ? vk : -vk;
+                part = (vn < 0.0) ? 0.0 : wk * vn * dh;
+            }
+            }
+            for (int m = T >> 1; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
+            if (act && lane == 0) {
+                // a one-row grid has both walls on the same cells
+                if (j == 0) D.wallpart[((size_t)blockIdx.x * 2 + 0) * nx + i] = part;
+                if (j == ny - 1) D.wallpart[((size_t)blockIdx.x * 2 + 1) * nx + i] = part;
             }
         }
         __syncthreads();
@@ -382,7 +392,7 @@ __global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
 // backward sweep with the apply step: f' = f + delta (alpha = 1), per-tile
 // moment partials of f' for the macro recovery
 __global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
-    extern __shared__ double sm[];   // [2][2][nx][T] + [2][nx]
+    UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T] + [2][nx]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
     const int lane = threadIdx.x % T, slot = threadIdx.x / T, nslot = blockDim.x / T;
     const int k = blockIdx.x * T + lane;
@@ -412,9 +422,13 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
         const int d = nd - 1 - dd;
         double* cur = buf[dd & 1];
         const double* prv = buf[(dd + 1) & 1];
-        const int ilo = max(0, d - (ny - 1)), ihi = min(nx - 1, d);
-        for (int i = ilo + slot; i <= ihi; i += nslot) {
+        const int ilo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, ihi = d < nx - 1 ? d : nx - 1;
+        for (int base = ilo; base <= ihi; base += nslot) {
+            const int i = base + slot;
+            const bool act = i <= ihi;
             const int j = d - i, c = j * nx + i;
+            double m[4] = {0, 0, 0, 0};
+            if (act) {
             const size_t o = (size_t)c * nvel + k;
             const double tt = D.tau_t[c];
             double rh = D.rh[o], rb = D.rb[o];
@@ -466,11 +480,11 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
             fin = fin && isfinite(hn) && isfinite(bn);
             D.hn[o] = hn;
             D.bn[o] = bn;
-            double m[4] = {0, 0, 0, 0};
             mom_add(m, wk, uk, vk, hn, bn);
+            }
             for (int q = 0; q < 4; ++q)
                 for (int s = T >> 1; s >= 1; s >>= 1) m[q] += __shfl_xor(m[q], s, 64);
-            if (lane == 0)
+            if (act && lane == 0)
                 for (int q = 0; q < 4; ++q) D.mompart[((size_t)blockIdx.x * D.ncell + c) * 4 + q] = m[q];
         }
         __syncthreads();

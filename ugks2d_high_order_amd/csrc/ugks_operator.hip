@@ -40,10 +40,12 @@ __global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vma
 // ===========================================================================
 // macro (W) reconstruction: face slopes and volume nodes, unlimited
 // ===========================================================================
+// One workgroup row per cell row (blockIdx.y = j), so the wall-stencil choice
+// is uniform over the workgroup and the near-wall branch never diverges.
 __global__ void __launch_bounds__(256) k_wrecon(Dev D) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 4 * D.ncell) return;
-    int c = t >> 2, m = t & 3, i = c % D.nx, j = c / D.nx;
+    if (t >= 4 * D.nx) return;
+    const int j = blockIdx.y, i = t >> 2, m = t & 3, c = j * D.nx + i;
     int rb, sh;
     bool near;
     stencil_rows(j, D.ny, &rb, &near, &sh);
@@ -489,7 +491,7 @@ __global__ void __launch_bounds__(NT) k_face_flux(Dev D) {
                                 fw_acc, lds);
     }
     if (st) {
-        if (threadIdx.x == 0) atomicOr(D.status, st);
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_FACE);
         return;
     }
     double *oh, *ob, *ow;
@@ -619,7 +621,7 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
     double th[4], J[16], mh[KPT];
     if (st == 0 && !closure_solve<KPT, NT>(ck, ww, V, th, J, mh, lds)) st = ST_NEWTON;
     if (st) {
-        if (threadIdx.x == 0) atomicOr(D.status, st);
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_WALL);
         return;
     }
     double mb[KPT];
@@ -717,7 +719,7 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
         for (int q = 0; q < 4; ++q) fw_acc[q] += KC.gw[a] * fw[q];
     }
     if (st) {
-        if (threadIdx.x == 0) atomicOr(D.status, st);
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_WALL);
         return;
     }
     size_t fo = (size_t)(top ? ny : 0) * nx + i;
@@ -798,7 +800,7 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
             }
         }
     if (st) {
-        if (threadIdx.x == 0) atomicOr(D.status, st);
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
     }
     // physical Shakhov collision of the current state
@@ -872,7 +874,7 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
         }
         cp[12] = area * area;
         cp[13] = rw[0] * area;
-        if (st) atomicOr(D.status, st);
+        if (st) atomicOr(D.status, st | ST_AT_CELL);
     }
 }
 
@@ -945,7 +947,7 @@ __global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux,
     double w[4] = {rho, rho * ux, 0.0, 0.5 * rho / (lam * (D.gas.gamma - 1.0)) + 0.5 * rho * (ux * ux)};
     double th[4], J[16], qh[KPT];
     if (!closure_solve<KPT, NT>(D.gas.ck, w, V, th, J, qh, lds)) {
-        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)ST_NEWTON);
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_NEWTON | ST_AT_COLD));
         return;
     }
     for (int c = blockIdx.x; c < D.ncell; c += gridDim.x) {
@@ -1035,7 +1037,7 @@ extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vm
                                            hipStream_t s) {
     const Dev& D = *Dp;
     hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax);
-    hipLaunchKernelGGL(k_wrecon, dim3((4 * D.ncell + 255) / 256), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_recon, dim3((D.nvel + 127) / 128, D.ncell), dim3(128), 0, s, D);
     switch (D.nvel) {
     case 256: launch_closure_kernels<4, 64>(D, s); launch_cell_kernel<4, 64>(D, s); break;
