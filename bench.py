@@ -1,0 +1,191 @@
+"""Throughput of the HighOrderImplicit pseudo-time iteration on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|shipped]
+                    [--no-cpu] [--copy-calibration]
+
+A step is one pseudo-time iteration of the Couette solver (physical operator:
+time step, WENO5-AO reconstruction, UGKS face and wall fluxes, volume targets,
+Shakhov collision, residual norms; then the implicit update: PR(2) macro
+Gauss-Seidel, predicted equilibrium, one LU-SGS forward/backward pair and the
+closed-Couette mass rescale), i.e. Solve_UGKS's HighOrderImplicit iteration
+(reference src/mod_Solve.f90:115-165) with the operator of each accepted state
+reused for the next update.  The state starts as the cold-start equilibrium and
+stays resident in HBM; every step runs the full path.
+
+Prints one JSON line.  `value` is FP64 cell-velocity updates per second
+(nx*ny*Nu*Nv per iteration); `roofline` covers the dominant kernel, timed with
+HIP events on the library's stream over the timed steps; `cpu_baseline` times
+the C oracle on the host cores over a bounded slab of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic HBM bytes of one iteration per cell-velocity: read h,b and write
+# the streaming+collision rates (4 cv-arrays), read h,b,rates and write the
+# forward corrections (6), read h,b,rates,corrections and write the new h,b
+# (8): 18 FP64 cv-arrays.
+HOI_BYTES_PER_CV = 18 * 8
+
+# FP64 operations per face-velocity of k_face_flux (a face-velocity is one
+# velocity of one of the ncell + nx*(ny-1) interior faces): ADD + MUL + 2*FMA
+# + TRANS F64 VALU instructions x 64 lanes from a rocprofv3 --pmc pass over
+# this workload, divided by the face-velocities of the launch.
+FACE_FLUX_FLOP_PER_FACE_VELOCITY = None
+
+MI355X_HBM_GBPS = 8000.0
+MI355X_FP64_TFLOPS = 78.6
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["large", "shipped"], default="large")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for scaling FETCH_SIZE/WRITE_SIZE counters)")
+    return ap.parse_args()
+
+
+def _case(name):
+    from ugks2d_high_order_amd import config
+    if name == "large":
+        case = config.couette_large()
+        desc = ("HighOrderImplicit Couette, 64x512 cells x 64x64 velocities (half-range Gauss-Hermite 64, scale 2), "
+                "Ma=3 diffuse walls, Kn=0.05, Shakhov Pr=2/3, omega=0.81, PR(2) + one LU-SGS pair, mass rescale")
+    else:
+        case = config.couette_ma3_kn01()
+        desc = ("HighOrderImplicit Couette, 8x64 cells x 32x32 velocities (half-range Gauss-Hermite 32, scale 2), "
+                "Ma=3 diffuse walls, Kn=0.1, Shakhov Pr=2/3, omega=0.81, PR(2) + one LU-SGS pair, mass rescale")
+    return case, desc
+
+
+def cpu_baseline(case, threads):
+    """Oracle (C, OpenMP) on the host: one untimed iteration, then timed
+    iterations (operator + implicit update) on a slab of the workload with the
+    same x grid and velocity grid and 16 cell rows."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from ugks2d_high_order_amd import velocity
+    oracle.build()
+    slab = dataclasses.replace(case, ny=16)
+    u, v, w = velocity.table(slab.table, slab.table_scale)
+    o = oracle.OracleSolver(slab, u, v, w, threads=threads)
+    o.cold_start(1.0, 0.0, slab.t_init)
+    st, acc, _ = o.iteration()
+    if st or not acc:
+        raise RuntimeError(f"oracle iteration failed ({st})")
+    n, t0 = 0, time.perf_counter()
+    while True:
+        st, acc, _ = o.iteration()
+        if st or not acc:
+            raise RuntimeError(f"oracle iteration failed ({st})")
+        n += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or n >= 4:
+            break
+    cv = slab.nx * slab.ny * slab.nu * slab.nv * n
+    return {"value": cv / el, "unit": "cell-velocity updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} pseudo-time iterations (operator + implicit update) of a {slab.nx}x{slab.ny}-cell slab "
+                      f"with the same {slab.nu}x{slab.nv} velocity grid and physics, after one untimed iteration; "
+                      f"{el:.1f} s on {threads} OpenMP threads"}
+
+
+def copy_calibration():
+    import torch
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}))
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the y-slab sharding "
+              "over several GPUs is described, not built)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    if a.copy_calibration:
+        copy_calibration()
+        return
+    import ugks2d_high_order_amd as U
+    from ugks2d_high_order_amd import velocity
+    case, desc = _case(a.workload)
+    u, v, w = velocity.table(case.table, case.table_scale)
+    g = U.UGKS(case, u, v, w)
+    g.cold_start()
+    for _ in range(a.warmup):
+        acc, _d = g.implicit_step()
+        if not acc:
+            raise RuntimeError("warm-up iteration rejected")
+    g.sync()
+    torch.cuda.synchronize()
+    g.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        acc, _d = g.implicit_step()
+        if not acc:
+            raise RuntimeError("iteration rejected")
+    g.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g.profile(False)
+    kt = g.profile_read()
+    el = t1 - t0
+    cv = case.nx * case.ny * case.nu * case.nv
+    value = cv * a.steps / el
+
+    # dominant kernel: largest summed event time
+    name = max(kt, key=lambda k: kt[k][0])
+    ms, launches = kt[name]
+    avg_s = ms / max(launches, 1) / 1e3
+    if name == "k_face_flux" and FACE_FLUX_FLOP_PER_FACE_VELOCITY is not None:
+        units = (case.nx * case.ny + case.nx * (case.ny - 1)) * case.nu * case.nv
+        ach = FACE_FLUX_FLOP_PER_FACE_VELOCITY * units / avg_s / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64 valu (no dense contraction, no MFMA)", "kernel": name,
+                "achieved": ach, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_TFLOPS,
+                "traffic": None, "avg_launch_ms": avg_s * 1e3}
+    else:
+        ach = HOI_BYTES_PER_CV * cv / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
+                "frac": ach / MI355X_HBM_GBPS, "traffic": None, "avg_launch_ms": avg_s * 1e3}
+    kernels = {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0]) if v[1]}
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+        cpu = cpu_baseline(case, threads)
+
+    out = {
+        "metric": "FP64 cell-velocity updates/s", "value": value, "unit": "cell-velocity updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "hbm_GBps_algorithmic": HOI_BYTES_PER_CV * value / 1e9,
+        "config": {"workload": desc, "nx": case.nx, "ny": case.ny, "nu": case.nu, "nv": case.nv,
+                   "iterations_timed": a.steps},
+        "roofline": roof, "cpu_baseline": cpu, "kernel_avg_ms": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

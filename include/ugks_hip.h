@@ -110,6 +110,20 @@ int ugks_sync(void* handle);
  * 4 macro face slopes, 5 macro volume nodes, 6 device scalars) to out. */
 int ugks_debug_fetch(void* handle, int which, double* out, long n);
 
+/* Kernel timing (no counterpart in the reference): ugks_profile(h, 1) clears
+ * the totals and brackets every later launch with HIP events on the
+ * instance's stream; ugks_profile(h, 0) stops.  ugks_profile_read returns,
+ * for kernel id idx (0 .. until it returns 7), its name, the summed event
+ * time in ms and the number of launches. */
+int ugks_profile(void* handle, int enable);
+int ugks_profile_read(void* handle, int idx, const char** name, double* total_ms, long* launches);
+
+/* Launch-shape tuning (no counterpart in the reference).  Keys:
+ *   1  workgroup size of the velocity-parallel operator kernels (face, wall,
+ *      cell, cold start): 64, 128 or 256. */
+int ugks_set_tuning(void* handle, int key, int value);
+int ugks_get_tuning(void* handle, int key, int* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@ EXPORTS = [
     "ugks_create", "ugks_destroy", "ugks_last_error", "ugks_set_state", "ugks_get_state", "ugks_cold_start",
     "ugks_evolution", "ugks_norms", "ugks_implicit_step", "ugks_run", "ugks_observables",
     "ugks_set_state_device", "ugks_get_state_device", "ugks_sync", "ugks_debug_fetch",
+    "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -73,6 +74,11 @@ def load_library(path: str = LIB_PATH):
     L.ugks_observables.argtypes = [C.c_void_p, _D]
     L.ugks_sync.argtypes = [C.c_void_p]
     L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
+    L.ugks_profile.argtypes = [C.c_void_p, C.c_int]
+    L.ugks_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ugks_get_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.ugks_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_long)]
     _lib = L
     return L
 
@@ -191,6 +197,30 @@ class UGKS:
         out = np.zeros(n)
         _check(load_library().ugks_debug_fetch(self.handle, int(which), _ptr(out), int(n)))
         return out
+
+    def profile(self, enable: bool):
+        """Start (clearing the totals) or stop HIP-event timing of every kernel launch."""
+        _check(load_library().ugks_profile(self.handle, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} since profiling was enabled."""
+        L = load_library()
+        out, idx = {}, 0
+        name, ms, n = C.c_char_p(), C.c_double(), C.c_long()
+        while L.ugks_profile_read(self.handle, idx, C.byref(name), C.byref(ms), C.byref(n)) == 0:
+            out[name.value.decode()] = (ms.value, n.value)
+            idx += 1
+        return out
+
+    TUNE_VEL_WG = 1
+
+    def set_tuning(self, key: int, value: int):
+        _check(load_library().ugks_set_tuning(self.handle, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        v = C.c_int(0)
+        _check(load_library().ugks_get_tuning(self.handle, int(key), C.byref(v)))
+        return v.value
 
     def sync(self):
         _check(load_library().ugks_sync(self.handle))

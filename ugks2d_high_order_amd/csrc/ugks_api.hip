@@ -15,11 +15,13 @@
 #include <vector>
 
 #include "ugks_state.h"
+#include "ugks_prof.h"
 
 extern "C" hipError_t ugks_set_consts(const Consts* k);
-extern "C" hipError_t ugks_launch_operator(const Dev* D, double umax, double vmax, double residual_tol, hipStream_t s);
-extern "C" hipError_t ugks_launch_implicit(const Dev* D, int rescale, hipStream_t s);
-extern "C" hipError_t ugks_launch_wall_reflection(const Dev* D, hipStream_t s);
+extern "C" hipError_t ugks_launch_operator(const Dev* D, double umax, double vmax, double residual_tol, hipStream_t s,
+                                           Prof* P);
+extern "C" hipError_t ugks_launch_implicit(const Dev* D, int rescale, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_wall_reflection(const Dev* D, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
 extern "C" hipError_t ugks_launch_observables(const Dev* D, double* out, hipStream_t s);
 
@@ -48,6 +50,7 @@ struct Instance {
     bool operator_fresh = false;
     std::vector<void*> allocs;
     double* obs = nullptr;
+    Prof prof;
 };
 
 // Polynomial coefficients (degrees 0..n-1) whose averages over the unit cells
@@ -154,7 +157,7 @@ int check_status(Instance* I, unsigned* out_status) {
 }
 
 int run_operator(Instance* I) {
-    HIPCHK(ugks_launch_operator(&I->D, I->umax, I->vmax, I->p.residual_tol, I->stream));
+    HIPCHK(ugks_launch_operator(&I->D, I->umax, I->vmax, I->p.residual_tol, I->stream, &I->prof));
     int rc = check_status(I, nullptr);
     if (rc) return rc;
     I->operator_fresh = true;
@@ -191,6 +194,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     D.wall_tn[0] = p->wall_tn[0]; D.wall_tn[1] = p->wall_tn[1];
     D.macro_sweeps = p->macro_sweeps;
     D.tile = 16;
+    D.wg_vel = 64;
     D.ntiles = nvel / D.tile;
     I->umax = -1e300; I->vmax = -1e300;
     for (int k = 0; k < nvel; ++k) { I->umax = std::max(I->umax, u[k]); I->vmax = std::max(I->vmax, v[k]); }
@@ -219,7 +223,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
         hipMemcpy(dv, v, sizeof(double) * nvel, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dw, w, sizeof(double) * nvel, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(E_DEVICE, "velocity table upload failed"));
-    if (ugks_launch_wall_reflection(&D, I->stream) != hipSuccess) return bail(fail(E_DEVICE, "launch failed"));
+    if (ugks_launch_wall_reflection(&D, I->stream, &I->prof) != hipSuccess) return bail(fail(E_DEVICE, "launch failed"));
     if ((rc = check_status(I, nullptr))) return bail(rc);
     *handle = I;
     return E_OK;
@@ -229,6 +233,7 @@ extern "C" void ugks_destroy(void* handle) {
     Instance* I = (Instance*)handle;
     if (!I) return;
     if (I->stream) hipStreamSynchronize(I->stream);
+    I->prof.release();
     for (void* p : I->allocs) hipFree(p);
     if (I->stream) hipStreamDestroy(I->stream);
     delete I;
@@ -327,7 +332,7 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
         return fail(E_POLICY, "closed-Couette policy requires nCFL = 1000 (CheckR5Policy)");
     int rc = E_OK;
     if (!I->operator_fresh && (rc = run_operator(I))) return rc;
-    HIPCHK(ugks_launch_implicit(&I->D, p.mass_rescale ? 1 : 0, I->stream));
+    HIPCHK(ugks_launch_implicit(&I->D, p.mass_rescale ? 1 : 0, I->stream, &I->prof));
     double sc[SC_COUNT];
     unsigned st = 0;
     HIPCHK(hipMemcpyAsync(sc, I->D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
@@ -386,6 +391,56 @@ extern "C" int ugks_debug_fetch(void* handle, int which, double* out, long n) {
     HIPCHK(hipMemcpyAsync(out, src[which], sizeof(double) * m, hipMemcpyDeviceToHost, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));
     return E_OK;
+}
+
+// Kernel timing (no counterpart in the reference): enable != 0 clears the
+// totals and brackets every following launch with HIP events on the
+// instance's stream; enable == 0 stops recording and keeps the totals.
+extern "C" int ugks_profile(void* handle, int enable) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(I->stream));
+    if (enable) I->prof.reset();
+    else I->prof.collect();
+    I->prof.on = enable != 0;
+    return E_OK;
+}
+
+// Totals of kernel `idx` since profiling was enabled: name, summed event
+// time in milliseconds and number of launches.  Returns 7 past the last id.
+extern "C" int ugks_profile_read(void* handle, int idx, const char** name, double* total_ms, long* launches) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    if (idx < 0 || idx >= K_COUNT) return fail(E_ARG, "kernel index out of range");
+    HIPCHK(hipStreamSynchronize(I->stream));
+    I->prof.collect();
+    if (name) *name = kernel_name(idx);
+    if (total_ms) *total_ms = I->prof.total_ms[idx];
+    if (launches) *launches = I->prof.launches[idx];
+    return E_OK;
+}
+
+// Launch-shape tuning (no counterpart in the reference).  key 1: workgroup
+// size of the velocity-parallel operator kernels (64, 128 or 256).
+extern "C" int ugks_set_tuning(void* handle, int key, int value) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    switch (key) {
+    case 1:
+        if (value != 64 && value != 128 && value != 256) return fail(E_ARG, "velocity workgroup must be 64, 128 or 256");
+        I->D.wg_vel = value;
+        return E_OK;
+    default: return fail(E_ARG, "unknown tuning key");
+    }
+}
+
+extern "C" int ugks_get_tuning(void* handle, int key, int* value) {
+    Instance* I = (Instance*)handle;
+    if (!I || !value) return fail(E_ARG, "null argument");
+    switch (key) {
+    case 1: *value = I->D.wg_vel; return E_OK;
+    default: return fail(E_ARG, "unknown tuning key");
+    }
 }
 
 extern "C" int ugks_observables(void* handle, double* fields10) {

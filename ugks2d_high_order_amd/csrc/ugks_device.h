@@ -48,56 +48,91 @@ struct Consts {
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+#ifndef UGKS_EMU_REDUCTIONS
+// Lane moves through the DPP crossbar, on the two 32-bit halves of a double.
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double readlane_f64(double x, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+// DPP controls: quad_perm [1,0,3,2] and [2,3,0,1] (lane xor 1, xor 2), row
+// half-mirror (xor of the quads within 8 lanes), row mirror (halves of 16).
+// Each step adds a value and its partner, and a+b == b+a exactly, so after
+// the four steps every lane of a 16-lane row holds the identical row sum.
+enum { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140 };
+
+__device__ inline double row16_sum(double x) {
+    x += dpp_f64<DPP_XOR1>(x);
+    x += dpp_f64<DPP_XOR2>(x);
+    x += dpp_f64<DPP_HALF_MIRROR>(x);
+    x += dpp_f64<DPP_MIRROR>(x);
     return x;
+}
+__device__ inline double row16_max(double x) {
+    x = fmax(x, dpp_f64<DPP_XOR1>(x));
+    x = fmax(x, dpp_f64<DPP_XOR2>(x));
+    x = fmax(x, dpp_f64<DPP_HALF_MIRROR>(x));
+    x = fmax(x, dpp_f64<DPP_MIRROR>(x));
+    return x;
+}
+// wavefront (64 lanes) totals, identical in every lane
+__device__ inline double wave_sum(double x) {
+    x = row16_sum(x);
+    return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
 }
 __device__ inline double wave_max(double x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x = fmax(x, __shfl_xor(x, m, 64));
-    return x;
+    x = row16_max(x);
+    return fmax(fmax(readlane_f64(x, 0), readlane_f64(x, 16)), fmax(readlane_f64(x, 32), readlane_f64(x, 48)));
 }
 
-#ifndef UGKS_EMU_REDUCTIONS
-// Sum R values over the whole workgroup; every thread gets the totals.
-// lds must hold (blockDim.x/64)*R doubles.
+// Sum R <= 64 values over the whole workgroup; every thread gets the totals,
+// bitwise identical in all threads.  lds must hold (blockDim.x/16)*R doubles.
+// Each 16-lane row reduces through DPP; row leaders stage the row totals in
+// LDS; lane r of every wavefront then adds the row totals of value r in row
+// order, and the R totals are broadcast from lanes 0..R-1.
 template <int R>
 __device__ inline void block_sum(double* v, double* lds) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 4, nrow = blockDim.x >> 4;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = wave_sum(v[r]);
+    for (int r = 0; r < R; ++r) v[r] = row16_sum(v[r]);
     __syncthreads();
-    if (lane == 0) {
+    if ((lane & 15) == 0) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[wid * R + r] = v[r];
+        for (int r = 0; r < R; ++r) lds[row * R + r] = v[r];
     }
     __syncthreads();
+    double s = 0.0;
+    if (lane < R)
+        for (int w = 0; w < nrow; ++w) s += lds[w * R + lane];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += lds[w * R + r];
-        v[r] = s;
-    }
+    for (int r = 0; r < R; ++r) v[r] = readlane_f64(s, r);
 }
 
 template <int R>
 __device__ inline void block_max(double* v, double* lds) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 4, nrow = blockDim.x >> 4;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = wave_max(v[r]);
+    for (int r = 0; r < R; ++r) v[r] = row16_max(v[r]);
     __syncthreads();
-    if (lane == 0) {
+    if ((lane & 15) == 0) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[wid * R + r] = v[r];
+        for (int r = 0; r < R; ++r) lds[row * R + r] = v[r];
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        double s = lds[r];
-        for (int w = 1; w < nw; ++w) s = fmax(s, lds[w * R + r]);
-        v[r] = s;
+    double s = -1.0e308;
+    if (lane < R) {
+        s = lds[lane];
+        for (int w = 1; w < nrow; ++w) s = fmax(s, lds[w * R + lane]);
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = readlane_f64(s, r);
 }
 
 #endif

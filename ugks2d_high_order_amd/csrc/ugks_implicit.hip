@@ -17,6 +17,7 @@
 // wall closure of a cell is frozen from the previous sweep), so the
 // wavefront runs inside velocity-tile workgroups with no grid-wide barrier.
 #include "ugks_state.h"
+#include "ugks_prof.h"
 
 // ===========================================================================
 // PR(2) macro solver
@@ -214,7 +215,7 @@ __device__ inline void gram_full(const double G[10], double M[16]) {
 
 template <int KPT, int NT>
 __global__ void __launch_bounds__(NT) k_predict(Dev D) {
-    __shared__ double lds[(NT / 64) * 20];
+    __shared__ double lds[(NT / 16) * 20];
     const int c = blockIdx.x, nvel = D.nvel;
     const Gas& g = D.gas;
     double u[KPT], v[KPT], w[KPT], h[KPT], b[KPT];
@@ -494,7 +495,7 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
 
 // reflection normalisers of the frozen wall closure (depend on the table only)
 __global__ void __launch_bounds__(256) k_wall_reflection(Dev D) {
-    __shared__ double lds[8];
+    __shared__ double lds[16 * 4];
     double s[4] = {0, 0, 0, 0};
     for (int k = threadIdx.x; k < D.nvel; k += blockDim.x) {
         for (int top = 0; top < 2; ++top) {
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(256) k_recover(Dev D) {
 }
 
 __global__ void __launch_bounds__(256) k_mass(Dev D, int rescale) {
-    __shared__ double lds[8];
+    __shared__ double lds[16 * 1];
     double s[1] = {0.0}, s0[1] = {0.0};
     const double area = D.dx * D.dy;
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
@@ -558,36 +559,36 @@ __global__ void __launch_bounds__(256) k_rescale(Dev D) {
 }
 
 __global__ void __launch_bounds__(256) k_mass_after(Dev D) {
-    __shared__ double lds[8];
+    __shared__ double lds[16 * 1];
     double s[1] = {0.0};
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) s[0] += D.dx * D.dy * D.Wn[4 * c];
     block_sum<1>(s, lds);
     if (threadIdx.x == 0) D.scal[SC_MASS_AFTER] = s[0];
 }
 
-extern "C" hipError_t ugks_launch_wall_reflection(const Dev* Dp, hipStream_t s) {
-    hipLaunchKernelGGL(k_wall_reflection, dim3(1), dim3(256), 0, s, *Dp);
+extern "C" hipError_t ugks_launch_wall_reflection(const Dev* Dp, hipStream_t s, Prof* P) {
+    UGKS_TIMED(P, K_WALL_REFLECTION, s, hipLaunchKernelGGL(k_wall_reflection, dim3(1), dim3(256), 0, s, *Dp));
     return hipGetLastError();
 }
 
-extern "C" hipError_t ugks_launch_implicit(const Dev* Dp, int rescale, hipStream_t s) {
+extern "C" hipError_t ugks_launch_implicit(const Dev* Dp, int rescale, hipStream_t s, Prof* P) {
     const Dev& D = *Dp;
     int ng = (D.ncell + (D.ny + 1) * D.nx + 255) / 256;
-    hipLaunchKernelGGL(k_macro_gamma, dim3(ng), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_macro_pr2, dim3(1), dim3(256), 0, s, D);
+    UGKS_TIMED(P, K_MACRO_GAMMA, s, hipLaunchKernelGGL(k_macro_gamma, dim3(ng), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_MACRO_PR2, s, hipLaunchKernelGGL(k_macro_pr2, dim3(1), dim3(256), 0, s, D));
     switch (D.nvel) {
-    case 256: hipLaunchKernelGGL((k_predict<4, 64>), dim3(D.ncell), dim3(64), 0, s, D); break;
-    case 1024: hipLaunchKernelGGL((k_predict<4, 256>), dim3(D.ncell), dim3(256), 0, s, D); break;
-    case 4096: hipLaunchKernelGGL((k_predict<8, 512>), dim3(D.ncell), dim3(512), 0, s, D); break;
+    case 256: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 64>), dim3(D.ncell), dim3(64), 0, s, D)); break;
+    case 1024: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 256>), dim3(D.ncell), dim3(256), 0, s, D)); break;
+    case 4096: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<8, 512>), dim3(D.ncell), dim3(512), 0, s, D)); break;
     default: return hipErrorInvalidValue;
     }
     size_t shf = sizeof(double) * 4 * D.nx * D.tile;
     size_t shb = shf + sizeof(double) * 2 * D.nx;
-    hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles), dim3(256), shf, s, D);
-    hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles), dim3(256), shb, s, D);
-    hipLaunchKernelGGL(k_recover, dim3((D.ncell + 255) / 256), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_mass, dim3(1), dim3(256), 0, s, D, rescale);
-    hipLaunchKernelGGL(k_rescale, dim3(1024), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_mass_after, dim3(1), dim3(256), 0, s, D);
+    UGKS_TIMED(P, K_SWEEP_FWD, s, hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles), dim3(256), shf, s, D));
+    UGKS_TIMED(P, K_SWEEP_BWD, s, hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles), dim3(256), shb, s, D));
+    UGKS_TIMED(P, K_RECOVER, s, hipLaunchKernelGGL(k_recover, dim3((D.ncell + 255) / 256), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_MASS, s, hipLaunchKernelGGL(k_mass, dim3(1), dim3(256), 0, s, D, rescale));
+    UGKS_TIMED(P, K_RESCALE, s, hipLaunchKernelGGL(k_rescale, dim3(1024), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_MASS_AFTER, s, hipLaunchKernelGGL(k_mass_after, dim3(1), dim3(256), 0, s, D));
     return hipGetLastError();
 }

@@ -12,6 +12,7 @@
 //   TimeStep                                         src/mod_Solve.f90:373-421
 #include "ugks_recon.h"
 #include "ugks_state.h"
+#include "ugks_prof.h"
 
 __constant__ Consts KC;
 
@@ -19,7 +20,7 @@ __constant__ Consts KC;
 // time step (one workgroup)
 // ===========================================================================
 __global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vmax) {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double tmin = 1e300;
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
         double prim[4];
@@ -181,45 +182,45 @@ __global__ void __launch_bounds__(128) k_recon(Dev D) {
 // ===========================================================================
 // velocity-parallel closure machinery
 // ===========================================================================
-template <int KPT, int NT>
-struct VelR {
-    double u[KPT], v[KPT], w[KPT];
-};
+// Velocity loops: thread t of an NT-thread workgroup owns velocities t, t+NT,
+// t+2NT, ...  Per-velocity inputs (velocity table, records, states) are read
+// from memory in every pass and stay in the caches, so nothing per velocity is
+// held in registers across a workgroup reduction.  Per-velocity values shared
+// by several passes live in LDS slots indexed by velocity; a slot is only ever
+// touched by the thread that owns the velocity, so the slots need no barriers.
+#define UGKS_VLOOP(k, NT, n) _Pragma("unroll 1") for (int k = threadIdx.x; k < (n); k += (NT))
 
-template <int KPT, int NT>
-__device__ inline void load_vel(const Dev& D, VelR<KPT, NT>& V) {
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        V.u[r] = D.u[k]; V.v[r] = D.v[k]; V.w[r] = D.w[k];
-    }
-}
-
-// q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q, with the moments of q and
-// of dq/dtheta_a (Jacobian columns): red[0..3] moments, red[4+4a..] column a
-template <int KPT, int NT>
-__device__ inline void q_pass(int ck, const double th[4], const VelR<KPT, NT>& V, double qh[KPT], double red[20],
-                              double* lds) {
+// q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q: the moments of q and of
+// dq/dtheta_a (Jacobian columns), red[0..3] moments, red[4+4a..] column a.
+template <int NT>
+__device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds) {
 #pragma unroll
     for (int a = 0; a < 20; ++a) red[a] = 0.0;
     const double r0 = th[0], T = th[3];
     const double pre = r0 / (UGKS_PI * T);
     const double i0 = 1.0 / r0, iT = 1.0 / T;
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        double du = V.u[r] - th[1], dv = V.v[r] - th[2], c2 = du * du + dv * dv;
-        double q = pre * exp(-c2 / T);
-        qh[r] = q;
+    const int nvel = D.nvel;
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+        double q = pre * exp(-c2 * iT);
         double qb = 0.5 * (double)ck * T * q;
-        double l1 = 2.0 * du / T, l2 = 2.0 * dv / T, l3 = -1.0 / T + c2 / (T * T);
+        double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
         double lb3 = l3 + iT;
-        mom_add(red + 0, V.w[r], V.u[r], V.v[r], q, qb);
-        mom_add(red + 4, V.w[r], V.u[r], V.v[r], q * i0, qb * i0);
-        mom_add(red + 8, V.w[r], V.u[r], V.v[r], q * l1, qb * l1);
-        mom_add(red + 12, V.w[r], V.u[r], V.v[r], q * l2, qb * l2);
-        mom_add(red + 16, V.w[r], V.u[r], V.v[r], q * l3, qb * lb3);
+        mom_add(red + 0, w, u, v, q, qb);
+        mom_add(red + 4, w, u, v, q * i0, qb * i0);
+        mom_add(red + 8, w, u, v, q * l1, qb * l1);
+        mom_add(red + 12, w, u, v, q * l2, qb * l2);
+        mom_add(red + 16, w, u, v, q * l3, qb * lb3);
     }
     block_sum<20>(red, lds);
+}
+
+// q(theta) of one velocity (the expression q_pass uses); iT = 1/T.  Hot
+// loops multiply by reciprocals of the uniform T, rho instead of dividing.
+__device__ inline double q_of(const double th[4], double pre, double iT, double u, double v) {
+    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+    return pre * exp(-c2 * iT);
 }
 
 __device__ inline double scaled_res(const double r[4], const double w[4]) {
@@ -230,11 +231,13 @@ __device__ inline double scaled_res(const double r[4], const double w[4]) {
 }
 
 // Newton solve for the discrete conservative equilibrium with moments w
-// (halving line search down to 2^-20, at most 12 steps); on success th, J and
-// the owned q values describe the solution.
-template <int KPT, int NT>
-__device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, NT>& V, double th[4], double J[16],
-                                     double qh[KPT], double* lds) {
+// (halving line search down to 2^-20, at most 12 steps).  On success th and
+// J describe the solution and, with STORE, qh[k] (LDS slots of the owned
+// velocities) holds q(th), all positive and finite.
+// With STORE false the caller computes q itself and must check positivity.
+template <int NT, bool STORE>
+__device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], double th[4], double J[16], double* qh,
+                                     double* lds) {
     if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || !(w[0] > 0.0)) return false;
     th[0] = w[0];
     th[1] = w[1] / w[0];
@@ -242,7 +245,7 @@ __device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, 
     th[3] = 4.0 * (w[3] / w[0] - 0.5 * (th[1] * th[1] + th[2] * th[2])) / (double)(ck + 2);
     if (!(th[3] > 0.0) || !isfinite(th[3])) return false;
     double red[20], res[4];
-    q_pass<KPT, NT>(ck, th, V, qh, red, lds);
+    q_pass<NT>(ck, D, th, red, lds);
 #pragma unroll
     for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
 #pragma unroll
@@ -252,7 +255,9 @@ __device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, 
         if (nrm <= UGKS_NEWTON_TOL) break;
         double step[4], mres[4] = {-res[0], -res[1], -res[2], -res[3]};
         if (!solve4(J, mres, step)) return false;
-        double alpha = 1.0;
+        // line search: J and res are rebuilt from the accepted trial, so
+        // neither is live across the trial passes
+        double alpha = 1.0, nn = 0.0;
         bool acc = false;
         while (alpha >= 9.5367431640625e-07) {
             double nt[4];
@@ -260,8 +265,7 @@ __device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, 
             for (int a = 0; a < 4; ++a) nt[a] = th[a] + alpha * step[a];
             if (nt[0] > 0.0 && nt[3] > 0.0 && isfinite(nt[0]) && isfinite(nt[1]) && isfinite(nt[2]) &&
                 isfinite(nt[3])) {
-                double qt[KPT];
-                q_pass<KPT, NT>(ck, nt, V, qt, red, lds);
+                q_pass<NT>(ck, D, nt, red, lds);
                 bool fin = true;
 #pragma unroll
                 for (int a = 0; a < 20; ++a) fin = fin && isfinite(red[a]);
@@ -269,16 +273,11 @@ __device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, 
                     double nres[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) nres[a] = red[a] - w[a];
-                    double nn = scaled_res(nres, w);
+                    nn = scaled_res(nres, w);
                     if (nn < nrm || nn <= UGKS_NEWTON_TOL) {
                         acc = true;
 #pragma unroll
-                        for (int a = 0; a < 4; ++a) { th[a] = nt[a]; res[a] = nres[a]; }
-#pragma unroll
-                        for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
-#pragma unroll
-                        for (int r = 0; r < KPT; ++r) qh[r] = qt[r];
-                        nrm = nn;
+                        for (int a = 0; a < 4; ++a) th[a] = nt[a];
                     }
                 }
                 if (acc) break;
@@ -286,23 +285,33 @@ __device__ inline bool closure_solve(int ck, const double w[4], const VelR<KPT, 
             alpha *= 0.5;
         }
         if (!acc) return false;
+#pragma unroll
+        for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
+        nrm = nn;
     }
     if (!(nrm <= UGKS_NEWTON_TOL)) return false;
+    if (!STORE) return true;
+    const double pre = th[0] / (UGKS_PI * th[3]), iT = 1.0 / th[3];
     int pos = 1;
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) pos &= (qh[r] > 0.0 && isfinite(qh[r])) ? 1 : 0;
+    UGKS_VLOOP(k, NT, D.nvel) {
+        double q = q_of(th, pre, iT, D.u[k], D.v[k]);
+        qh[k] = q;
+        pos &= (q > 0.0 && isfinite(q)) ? 1 : 0;
+    }
     return __syncthreads_and(pos) != 0;
 }
 
-// dq/dtheta_a of the owned velocity r, h and b parts
-__device__ inline void qtheta(int ck, const double th[4], double u, double v, double q, double qt_h[4],
-                             double qt_b[4]) {
+// dq/dtheta_a of one velocity, h and b parts (i0 = 1/rho, iT = 1/T)
+__device__ inline void qtheta(int ck, const double th[4], double i0, double iT, double u, double v, double q,
+                             double qt_h[4], double qt_b[4]) {
     double T = th[3];
     double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
     double qb = 0.5 * (double)ck * T * q;
-    double l0 = 1.0 / th[0], l1 = 2.0 * du / T, l2 = 2.0 * dv / T, l3 = -1.0 / T + c2 / (T * T);
-    qt_h[0] = q * l0; qt_h[1] = q * l1; qt_h[2] = q * l2; qt_h[3] = q * l3;
-    qt_b[0] = qb * l0; qt_b[1] = qb * l1; qt_b[2] = qb * l2; qt_b[3] = qb * (l3 + 1.0 / T);
+    double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
+    qt_h[0] = q * i0; qt_h[1] = q * l1; qt_h[2] = q * l2; qt_h[3] = q * l3;
+    qt_b[0] = qb * i0; qt_b[1] = qb * l1; qt_b[2] = qb * l2; qt_b[3] = qb * (l3 + iT);
 }
 
 // S2 primitive (theta_gas = p/rho)
@@ -321,19 +330,20 @@ __device__ inline bool s2_prim(int ck, const double w[4], double* rho, double* u
 // ===========================================================================
 // UGKS interior face flux: one workgroup per face, three Gauss nodes
 // ===========================================================================
-template <int KPT, int NT>
-__device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int axis, double dt, const double f0h[KPT],
-                                     const double f0b[KPT], const double fnh[KPT], const double fnb[KPT],
-                                     const double fth[KPT], const double ftb[KPT], const double mnl[4],
-                                     const double mnr[4], const double mtg[4], double gwa, double acc_h[KPT],
-                                     double acc_b[KPT], double fw_acc[4], double* lds) {
-    const int ck = D.gas.ck;
+// One Gauss node.  rec: the node's upwind record, rows {h, h_n, h_t, b, b_n,
+// b_t} of nvel values.  The flux times the Gauss weight is added into oh/ob
+// (written on the first node), the macro flux into fw_acc.
+template <int NT>
+__device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const double* rec, const double* sl_p,
+                                     const double* sr_p, double ln, double lt, double gwa, bool first, double* oh,
+                                     double* ob, double fw_acc[4], double* qh, double* lds) {
+    const int ck = D.gas.ck, nvel = D.nvel;
+    const size_t nv = (size_t)nvel;
     double w0[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) mom_add(w0, V.w[r], V.u[r], V.v[r], f0h[r], f0b[r]);
+    UGKS_VLOOP(k, NT, nvel) mom_add(w0, D.w[k], D.u[k], D.v[k], rec[k], rec[3 * nv + k]);
     block_sum<4>(w0, lds);
-    double th[4], J[16], qh[KPT];
-    if (!closure_solve<KPT, NT>(ck, w0, V, th, J, qh, lds)) return ST_NEWTON;
+    double th[4], J[16];
+    if (!closure_solve<NT, true>(ck, D, w0, th, J, qh, lds)) return ST_NEWTON;
     double rho, ux, vy, thg;
     if (!s2_prim(ck, w0, &rho, &ux, &vy, &thg)) return ST_NONFINITE;
     const double lam = 0.5 / thg;
@@ -342,25 +352,34 @@ __device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int a
     const double kin = 0.5 * (ux * ux + vy * vy);
     const double tna = 4.0 * (w0[3] / rho - kin) / (double)(ck + 2);
     const double lama = 1.0 / tna;
-    const double T = th[3];
+    const double T = th[3], i0 = 1.0 / th[0], iT = 1.0 / th[3];
     double red[10];
 #pragma unroll
     for (int a = 0; a < 10; ++a) red[a] = 0.0;
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
-        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double mh = qh[k], mb = 0.5 * (double)ck * T * mh;
+        double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
         double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
-        mom_add(red + 0, V.w[r], V.u[r], V.v[r], coeff * cx * shh * mh, coeff * cx * shb * mb);
-        mom_add(red + 4, V.w[r], V.u[r], V.v[r], coeff * cy * shh * mh, coeff * cy * shb * mb);
-        double ca2 = (V.u[r] - ux) * (V.u[r] - ux) + (V.v[r] - vy) * (V.v[r] - vy);
-        double eah = rho * lama / UGKS_PI * exp(-lama * ca2);
+        mom_add(red + 0, w, u, v, coeff * cx * shh * mh, coeff * cx * shb * mb);
+        mom_add(red + 4, w, u, v, coeff * cy * shh * mh, coeff * cy * shb * mb);
+        double eah = rho * lama / UGKS_PI * exp(-lama * c2);
         double eab = 0.5 * (double)ck * tna * eah;
-        double dh = f0h[r] - eah, db = f0b[r] - eab;
-        red[8] += 0.5 * V.w[r] * cx * (c2 * dh + db);
-        red[9] += 0.5 * V.w[r] * cy * (c2 * dh + db);
+        double dh = rec[k] - eah, db = rec[3 * nv + k] - eab;
+        red[8] += 0.5 * w * cx * (c2 * dh + db);
+        red[9] += 0.5 * w * cy * (c2 * dh + db);
     }
     block_sum<10>(red, lds);
+    // macro face slopes of the two sides at this node (normal, tangential)
+    double mnl[4], mnr[4], mtg[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const double* pl = sl_p + (size_t)m * 24;
+        const double* pr = sr_p + (size_t)m * 24;
+        mnl[m] = pl[0] / ln;
+        mnr[m] = pr[0] / ln;
+        mtg[m] = 0.5 * (pl[1] + pr[1]) / lt;
+    }
     double tdx[4], tdy[4], tnl[4], tnr[4], ttg[4];
     if (!solve4(J, red + 0, tdx) || !solve4(J, red + 4, tdy) || !solve4(J, mnl, tnl) || !solve4(J, mnr, tnr) ||
         !solve4(J, mtg, ttg))
@@ -368,10 +387,10 @@ __device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int a
     const double q0x = red[8], q0y = red[9];
     // time slope moments
     double wt4[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
         double qth[4], qtb[4];
-        qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
+        qtheta(ck, th, i0, iT, u, v, qh[k], qth, qtb);
         double nlh = 0, nlb = 0, nrh = 0, nrb = 0, tgh = 0, tgb = 0;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -379,10 +398,10 @@ __device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int a
             nrh += qth[a] * tnr[a]; nrb += qtb[a] * tnr[a];
             tgh += qth[a] * ttg[a]; tgb += qtb[a] * ttg[a];
         }
-        double vn = axis == 1 ? V.u[r] : V.v[r], vt = axis == 1 ? V.v[r] : V.u[r];
+        double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
         double th_ = vn * (vn >= 0.0 ? nlh : nrh) + vt * tgh;
         double tb_ = vn * (vn >= 0.0 ? nlb : nrb) + vt * tgb;
-        mom_add(wt4, V.w[r], V.u[r], V.v[r], th_, tb_);
+        mom_add(wt4, w, u, v, th_, tb_);
     }
     block_sum<4>(wt4, lds);
     double wtime[4] = {-wt4[0], -wt4[1], -wt4[2], -wt4[3]}, ttm[4];
@@ -398,13 +417,15 @@ __device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int a
     mt[1] = -tau * mt[0] + mt[4];
     mt[2] = 0.5 * dt * dt - tau * mt[0];
     double fw[4] = {0, 0, 0, 0};
-    bool fin = true;
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
+    int fin = 1;
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        const double f0h = rec[k], fnh = rec[nv + k], fth = rec[2 * nv + k];
+        const double f0b = rec[3 * nv + k], fnb = rec[4 * nv + k], ftb = rec[5 * nv + k];
         double qth[4], qtb[4];
-        qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
-        double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
-        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
+        const double mh = qh[k], mb = 0.5 * (double)ck * T * mh;
+        qtheta(ck, th, i0, iT, u, v, mh, qth, qtb);
+        double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
         double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
         double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
         double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
@@ -422,94 +443,67 @@ __device__ inline unsigned ugks_node(const Dev& D, const VelR<KPT, NT>& V, int a
         kxh -= dxh; kxb -= dxb; kyh -= dyh; kyb -= dyb;
         double s0h = mh + kxh * q0x + kyh * q0y;
         double s0b = mb + kxb * q0x + kyb * q0y;
-        double vn = axis == 1 ? V.u[r] : V.v[r], vt = axis == 1 ? V.v[r] : V.u[r];
+        double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
         double slh = vn * (vn >= 0.0 ? nlh : nrh) + vt * tgh;
         double slb = vn * (vn >= 0.0 ? nlb : nrb) + vt * tgb;
-        double fh = vn * (mt[0] * s0h + mt[1] * slh + mt[2] * tmh + mt[3] * f0h[r] - mt[4] * (vn * fnh[r] + vt * fth[r]));
-        double fb = vn * (mt[0] * s0b + mt[1] * slb + mt[2] * tmb + mt[3] * f0b[r] - mt[4] * (vn * fnb[r] + vt * ftb[r]));
-        fin = fin && isfinite(fh) && isfinite(fb);
-        mom_add(fw, V.w[r], V.u[r], V.v[r], fh, fb);
-        acc_h[r] += gwa * fh;
-        acc_b[r] += gwa * fb;
+        double fh = vn * (mt[0] * s0h + mt[1] * slh + mt[2] * tmh + mt[3] * f0h - mt[4] * (vn * fnh + vt * fth));
+        double fb = vn * (mt[0] * s0b + mt[1] * slb + mt[2] * tmb + mt[3] * f0b - mt[4] * (vn * fnb + vt * ftb));
+        fin &= (isfinite(fh) && isfinite(fb)) ? 1 : 0;
+        mom_add(fw, w, u, v, fh, fb);
+        oh[k] = (first ? 0.0 : oh[k]) + gwa * fh;
+        ob[k] = (first ? 0.0 : ob[k]) + gwa * fb;
     }
     block_sum<4>(fw, lds);
 #pragma unroll
     for (int a = 0; a < 4; ++a) fw_acc[a] += gwa * fw[a];
-    if (!__syncthreads_and(fin ? 1 : 0)) return ST_NONFINITE;
+    if (!__syncthreads_and(fin)) return ST_NONFINITE;
     return 0;
 }
 
-template <int KPT, int NT>
+template <int NT>
 __global__ void __launch_bounds__(NT) k_face_flux(Dev D) {
-    __shared__ double lds[(NT / 64) * 20];
-    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    __shared__ double lds[(NT / 16) * 20];
+    UGKS_DYN_SHARED(double, qh);
+    const int nx = D.nx, nvel = D.nvel;
     const int f = blockIdx.x;
     const int nfx = D.ncell;
     int axis, cl, cr;
     const double* rec;
+    double *oh, *ob, *ow;
     if (f < nfx) {
         axis = 1;
         int j = f / nx, i = f % nx;
         cl = f;
         cr = j * nx + (i == nx - 1 ? 0 : i + 1);
         rec = D.xf + (size_t)f * 18 * nvel;
+        oh = D.xfl_h + (size_t)f * nvel; ob = D.xfl_b + (size_t)f * nvel; ow = D.xfw + 4 * (size_t)f;
     } else {
         axis = 2;
         int g = f - nfx;   // face between rows j and j+1, g = j*nx+i
         cl = g;
         cr = g + nx;
         rec = D.yf + (size_t)g * 18 * nvel;
+        size_t fo = (size_t)g + nx;   // south face of row j+1
+        oh = D.yfl_h + fo * nvel; ob = D.yfl_b + fo * nvel; ow = D.yfw + 4 * fo;
     }
-    VelR<KPT, NT> V;
-    load_vel<KPT, NT>(D, V);
     const double dt = D.scal[SC_DT];
-    double acc_h[KPT], acc_b[KPT], fw_acc[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) { acc_h[r] = 0.0; acc_b[r] = 0.0; }
+    double fw_acc[4] = {0, 0, 0, 0};
     const double ln = axis == 1 ? D.dx : D.dy, lt = axis == 1 ? D.dy : D.dx;
     const int sl = axis == 1 ? 1 : 3, sr = axis == 1 ? 0 : 2;
     unsigned st = 0;
+#pragma unroll 1
     for (int a = 0; a < 3 && st == 0; ++a) {
-        double f0h[KPT], f0b[KPT], fnh[KPT], fnb[KPT], fth[KPT], ftb[KPT];
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
-            int k = threadIdx.x + NT * r;
-            const double* p = rec + (size_t)(a * 6) * nvel + k;
-            f0h[r] = p[0]; fnh[r] = p[(size_t)nvel]; fth[r] = p[2 * (size_t)nvel];
-            f0b[r] = p[3 * (size_t)nvel]; fnb[r] = p[4 * (size_t)nvel]; ftb[r] = p[5 * (size_t)nvel];
-        }
-        double mnl[4], mnr[4], mtg[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const double* pl = D.ws + (size_t)(cl * 4 + m) * 24 + (sl * 3 + a) * 2;
-            const double* pr = D.ws + (size_t)(cr * 4 + m) * 24 + (sr * 3 + a) * 2;
-            mnl[m] = pl[0] / ln;
-            mnr[m] = pr[0] / ln;
-            mtg[m] = 0.5 * (pl[1] + pr[1]) / lt;
-        }
-        st = ugks_node<KPT, NT>(D, V, axis, dt, f0h, f0b, fnh, fnb, fth, ftb, mnl, mnr, mtg, KC.gw[a], acc_h, acc_b,
-                                fw_acc, lds);
+        const double* pl = D.ws + (size_t)(cl * 4) * 24 + (sl * 3 + a) * 2;
+        const double* pr = D.ws + (size_t)(cr * 4) * 24 + (sr * 3 + a) * 2;
+        st = ugks_node<NT>(D, axis, dt, rec + (size_t)(a * 6) * nvel, pl, pr, ln, lt, KC.gw[a], a == 0, oh, ob,
+                           fw_acc, qh, lds);
     }
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_FACE);
         return;
     }
-    double *oh, *ob, *ow;
-    if (axis == 1) {
-        oh = D.xfl_h + (size_t)f * nvel; ob = D.xfl_b + (size_t)f * nvel; ow = D.xfw + 4 * (size_t)f;
-    } else {
-        size_t fo = (size_t)(f - nfx) + nx;   // south face of row j+1
-        oh = D.yfl_h + fo * nvel; ob = D.yfl_b + fo * nvel; ow = D.yfw + 4 * fo;
-    }
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        oh[k] = acc_h[r];
-        ob[k] = acc_b[r];
-    }
     if (threadIdx.x == 0)
         for (int m = 0; m < 4; ++m) ow[m] = fw_acc[m];
-    (void)ny;
 }
 
 // ===========================================================================
@@ -594,16 +588,44 @@ __device__ inline void wall_time_coeffs(double dt, double tau, double mt[5]) {
     }
 }
 
-template <int KPT, int NT>
+// incident-side wall distribution of one velocity at one Gauss node
+__device__ inline void wall_gin(const double mt[5], const double an[4], const double at[4], const double att[4],
+                                const double mxi[3], double coeff, double lambda, double ux, int ck, bool shakhov,
+                                const double red[6], double u, double v, double mh, double mb, double hin, double hn,
+                                double ht, double bin, double bn, double bt, double* gh, double* gb) {
+    double vn = v, vt = -u, v2 = vn * vn + vt * vt;
+    double corrh = 0.0, corrb = 0.0;
+    if (shakhov) {
+        double vtl = -u + ux, v2l = vn * vn + vtl * vtl;
+        double qdot = vn * red[4] + vtl * red[5], shh = 2.0 * lambda * v2l - 4.0;
+        corrh = coeff * qdot * shh * mh;
+        corrb = coeff * qdot * (shh + 2.0 / (double)ck) * mb;
+    }
+    double Han = (an[0] + an[1] * vn + an[2] * vt) * mh + 0.5 * an[3] * (v2 * mh + mb);
+    double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh + 0.5 * at[3] * (v2 * mh + mb);
+    double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh + 0.5 * att[3] * (v2 * mh + mb);
+    double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb + 0.5 * an[3] * (v2 * mb + mxi[2] * mh);
+    double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb + 0.5 * at[3] * (v2 * mb + mxi[2] * mh);
+    double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb + 0.5 * att[3] * (v2 * mb + mxi[2] * mh);
+    double g_h = mt[0] * mh + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin - mt[4] * (vn * hn + u * ht);
+    double g_b = mt[0] * mb + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin - mt[4] * (vn * bn + u * bt);
+    *gh = g_h + mt[0] * corrh;
+    *gb = g_b + mt[0] * corrb;
+}
+
+template <int NT>
 __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
-    __shared__ double lds[(NT / 64) * 20];
+    __shared__ double lds[(NT / 16) * 20];
+    UGKS_DYN_SHARED(double, mhs);
     const int nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
+    const size_t nv = (size_t)nvel;
     const int top = blockIdx.x >= nx ? 1 : 0, i = blockIdx.x % nx;
     const int side = top ? 1 : -1;
     const int c = top ? (ny - 1) * nx + i : i;
     const double* rec = (top ? D.wt : D.wb) + (size_t)i * 18 * nvel;
-    VelR<KPT, NT> V;
-    load_vel<KPT, NT>(D, V);
+    const size_t fo = (size_t)(top ? ny : 0) * nx + i;
+    double* oh = D.yfl_h + fo * nv;
+    double* ob = D.yfl_b + fo * nv;
     const double dt = D.scal[SC_DT], dn = D.dy;
     const double ux = D.wall_ux[top], theta = 0.5 * D.wall_tn[top];
     const double* wc0 = D.W + 4 * c;
@@ -618,15 +640,13 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
     for (int a = 0; a < 4; ++a) sw[a] = (wg[a] - wc[a]) / ((double)side * 0.5 * dn);
     micro_slope(ck, prim_g, sw, an);
     double ww[4] = {prim_g[0], prim_g[0] * ux, 0.0, prim_g[0] * ((double)(ck + 2) * theta / 2.0 + ux * ux / 2.0)};
-    double th[4], J[16], mh[KPT];
-    if (st == 0 && !closure_solve<KPT, NT>(ck, ww, V, th, J, mh, lds)) st = ST_NEWTON;
+    double th[4], J[16];
+    if (st == 0 && !closure_solve<NT, true>(ck, D, ww, th, J, mhs, lds)) st = ST_NEWTON;
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_WALL);
         return;
     }
-    double mb[KPT];
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) mb[r] = 0.5 * (double)ck * th[3] * mh[r];
+    const double bfac = 0.5 * (double)ck * th[3];
     double mu[7], mv[6], mxi[3];
     half_moments(ck, prim_g, mu, mv, mxi);
     double wtau[4] = {wg[0], -wg[2], wg[1], wg[3]};
@@ -634,65 +654,43 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
     double mt[5];
     wall_time_coeffs(dt, tau_w, mt);
     const double coeff = 0.8 * (1.0 - D.gas.pr) * lambda * lambda / prim_g[0];
-    double acc_h[KPT], acc_b[KPT], fw_acc[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) { acc_h[r] = 0.0; acc_b[r] = 0.0; }
+    const double irg = 1.0 / prim_g[0];
+    const bool shakhov = D.gas.pr != 1.0;
+    double fw_acc[4] = {0, 0, 0, 0};
     for (int a = 0; a < 3 && st == 0; ++a) {
-        double hin[KPT], bin[KPT], hn[KPT], bn[KPT], ht[KPT], bt[KPT];
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
-            int k = threadIdx.x + NT * r;
-            const double* q = rec + (size_t)(a * 6) * nvel + k;
-            hin[r] = q[0]; hn[r] = q[(size_t)nvel]; ht[r] = q[2 * (size_t)nvel];
-            bin[r] = q[3 * (size_t)nvel]; bn[r] = q[4 * (size_t)nvel]; bt[r] = q[5 * (size_t)nvel];
-        }
+        const double* q = rec + (size_t)(a * 6) * nvel;
         double red[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
-            double vn = V.v[r], vt = -V.u[r], v2 = vn * vn + vt * vt;
-            red[0] += V.w[r] * (-ht[r]);
-            red[1] += V.w[r] * (-vn * ht[r]);
-            red[2] += V.w[r] * (-vt * ht[r]);
-            red[3] += V.w[r] * (-0.5 * (v2 * ht[r] + bt[r]));
-            double vtl = -V.u[r] + ux, v2l = vn * vn + vtl * vtl;
-            red[4] += 0.5 * V.w[r] * vn * (v2l * hin[r] + bin[r]);
-            red[5] += 0.5 * V.w[r] * vtl * (v2l * hin[r] + bin[r]);
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double hin = q[k], ht = q[2 * nv + k], bin = q[3 * nv + k], bt = q[5 * nv + k];
+            double vn = v, vt = -u, v2 = vn * vn + vt * vt;
+            red[0] += w * (-ht);
+            red[1] += w * (-vn * ht);
+            red[2] += w * (-vt * ht);
+            red[3] += w * (-0.5 * (v2 * ht + bt));
+            double vtl = -u + ux, v2l = vn * vn + vtl * vtl;
+            red[4] += 0.5 * w * vn * (v2l * hin + bin);
+            red[5] += 0.5 * w * vtl * (v2l * hin + bin);
         }
         block_sum<6>(red, lds);
         double at[4], att[4], m1[4], m2[4], sw2[4];
         micro_slope(ck, prim_g, red, at);
         m_auvxi(an, mu, mv, mxi, 1, 0, m1);
         m_auvxi(at, mu, mv, mxi, 0, 1, m2);
-        for (int q = 0; q < 4; ++q) sw2[q] = -prim_g[0] * (m1[q] + m2[q]);
+        for (int qq = 0; qq < 4; ++qq) sw2[qq] = -prim_g[0] * (m1[qq] + m2[qq]);
         micro_slope(ck, prim_g, sw2, att);
-        const bool shakhov = D.gas.pr != 1.0;
-        double ginh[KPT], ginb[KPT], ir[2] = {0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
-            double vn = V.v[r], vt = -V.u[r], v2 = vn * vn + vt * vt;
-            double corrh = 0.0, corrb = 0.0;
-            if (shakhov) {
-                double vtl = -V.u[r] + ux, v2l = vn * vn + vtl * vtl;
-                double qdot = vn * red[4] + vtl * red[5], shh = 2.0 * lambda * v2l - 4.0;
-                corrh = coeff * qdot * shh * mh[r];
-                corrb = coeff * qdot * (shh + 2.0 / (double)ck) * mb[r];
+        double ir[2] = {0.0, 0.0};
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double mh = mhs[k], mb = bfac * mh;
+            if ((double)side * v > 0.0) {
+                double gh, gb;
+                wall_gin(mt, an, at, att, mxi, coeff, lambda, ux, ck, shakhov, red, u, v, mh, mb, q[k], q[nv + k],
+                         q[2 * nv + k], q[3 * nv + k], q[4 * nv + k], q[5 * nv + k], &gh, &gb);
+                ir[0] += w * v * gh;
+            } else {
+                ir[1] += dt * w * v * (mh * irg);
             }
-            double Han = (an[0] + an[1] * vn + an[2] * vt) * mh[r] + 0.5 * an[3] * (v2 * mh[r] + mb[r]);
-            double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh[r] + 0.5 * at[3] * (v2 * mh[r] + mb[r]);
-            double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh[r] + 0.5 * att[3] * (v2 * mh[r] + mb[r]);
-            double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb[r] + 0.5 * an[3] * (v2 * mb[r] + mxi[2] * mh[r]);
-            double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb[r] + 0.5 * at[3] * (v2 * mb[r] + mxi[2] * mh[r]);
-            double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb[r] + 0.5 * att[3] * (v2 * mb[r] + mxi[2] * mh[r]);
-            ginh[r] = mt[0] * mh[r] + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin[r] -
-                      mt[4] * (vn * hn[r] + V.u[r] * ht[r]);
-            ginb[r] = mt[0] * mb[r] + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin[r] -
-                      mt[4] * (vn * bn[r] + V.u[r] * bt[r]);
-            ginh[r] = ginh[r] + mt[0] * corrh;
-            ginb[r] = ginb[r] + mt[0] * corrb;
-            if ((double)side * vn > 0.0)
-                ir[0] += V.w[r] * vn * ginh[r];
-            else
-                ir[1] += dt * V.w[r] * vn * (mh[r] / prim_g[0]);
         }
         block_sum<2>(ir, lds);
         double inc = ir[0], refl = ir[1];
@@ -700,50 +698,57 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
         double rhow = -inc / refl;
         if (!isfinite(rhow) || !(rhow > 0.0)) { st = ST_WALL; break; }
         double fw[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
+        const double gwa = KC.gw[a];
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double mh = mhs[k], mb = bfac * mh;
             double fh, fb;
-            if ((double)side * V.v[r] > 0.0) {
-                fh = V.v[r] * ginh[r];
-                fb = V.v[r] * ginb[r];
+            if ((double)side * v > 0.0) {
+                double gh, gb;
+                wall_gin(mt, an, at, att, mxi, coeff, lambda, ux, ck, shakhov, red, u, v, mh, mb, q[k], q[nv + k],
+                         q[2 * nv + k], q[3 * nv + k], q[4 * nv + k], q[5 * nv + k], &gh, &gb);
+                fh = v * gh;
+                fb = v * gb;
             } else {
-                fh = dt * V.v[r] * rhow * (mh[r] / prim_g[0]);
-                fb = dt * V.v[r] * rhow * (mb[r] / prim_g[0]);
+                fh = dt * v * rhow * (mh * irg);
+                fb = dt * v * rhow * (mb * irg);
             }
-            mom_add(fw, V.w[r], V.u[r], V.v[r], fh, fb);
-            acc_h[r] += KC.gw[a] * fh;
-            acc_b[r] += KC.gw[a] * fb;
+            mom_add(fw, w, u, v, fh, fb);
+            oh[k] = (a == 0 ? 0.0 : oh[k]) + gwa * fh;
+            ob[k] = (a == 0 ? 0.0 : ob[k]) + gwa * fb;
         }
         block_sum<4>(fw, lds);
         if (!(fabs(fw[0]) <= UGKS_GATE * fmax(1.0, fabs(inc)))) { st = ST_WALL; break; }
-        for (int q = 0; q < 4; ++q) fw_acc[q] += KC.gw[a] * fw[q];
+        for (int qq = 0; qq < 4; ++qq) fw_acc[qq] += gwa * fw[qq];
     }
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_WALL);
         return;
     }
-    size_t fo = (size_t)(top ? ny : 0) * nx + i;
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        D.yfl_h[fo * nvel + k] = acc_h[r];
-        D.yfl_b[fo * nvel + k] = acc_b[r];
-    }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) D.yfw[4 * fo + q] = fw_acc[q];
+        for (int qq = 0; qq < 4; ++qq) D.yfw[4 * fo + qq] = fw_acc[qq];
 }
 
 // ===========================================================================
 // cell kernel: volume targets (9 closures), physical collision, streaming
 // rates and the per-cell residual partial sums
 // ===========================================================================
-template <int KPT, int NT>
+// Per volume node the closure state and the projected Shakhov directions are
+// kept in LDS (uniform values); the averaged Maxwellian (mh, mb) is kept per
+// velocity in LDS slots.  The averaged Shakhov kernels are rebuilt per
+// velocity in the final pass from the node states (q recomputed from theta).
+enum { CN_TH = 0, CN_UX = 4, CN_VY = 5, CN_LAM = 6, CN_COEFF = 7, CN_TDX = 8, CN_TDY = 12, CN_WT = 16, CN_PRE = 17,
+       CN_IT = 18, CN_I0 = 19, CN_LEN = 20 };
+
+template <int NT>
 __global__ void __launch_bounds__(NT) k_cell(Dev D) {
-    __shared__ double lds[(NT / 64) * 20];
+    __shared__ double lds[(NT / 16) * 20];
+    __shared__ double node[9 * CN_LEN];
+    UGKS_DYN_SHARED(double, dyn);
     const int nx = D.nx, nvel = D.nvel, ck = D.gas.ck;
+    double* amh = dyn;
+    double* amb = dyn + nvel;
     const int c = blockIdx.x, i = c % nx, j = c / nx;
-    VelR<KPT, NT> V;
-    load_vel<KPT, NT>(D, V);
     const double* Wc = D.W + 4 * c;
     double W[4] = {Wc[0], Wc[1], Wc[2], Wc[3]};
     // volume nodes of W and their Gauss mean
@@ -753,107 +758,114 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
     for (int cc = 0; cc < 3; ++cc)
         for (int a = 0; a < 3; ++a)
             for (int m = 0; m < 4; ++m) avg[m] += KC.gw[a] * KC.gw[cc] * nodes[m][a * 3 + cc];
-    double amh[KPT], amb[KPT], akxh[KPT], akxb[KPT], akyh[KPT], akyb[KPT];
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) amh[r] = amb[r] = akxh[r] = akxb[r] = akyh[r] = akyb[r] = 0.0;
     unsigned st = 0;
-    for (int cc = 0; cc < 3 && st == 0; ++cc)
-        for (int a = 0; a < 3 && st == 0; ++a) {
-            double wq[4];
-            for (int m = 0; m < 4; ++m) wq[m] = nodes[m][a * 3 + cc] + W[m] - avg[m];
-            const double wt2 = KC.gw[a] * KC.gw[cc];
-            double th[4], J[16], qh[KPT];
-            if (!closure_solve<KPT, NT>(ck, wq, V, th, J, qh, lds)) { st = ST_NEWTON; break; }
-            double rho, ux, vy, thg;
-            if (!s2_prim(ck, wq, &rho, &ux, &vy, &thg)) { st = ST_NONFINITE; break; }
-            const double lam = 0.5 / thg, coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho, T = th[3];
-            double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int r = 0; r < KPT; ++r) {
-                double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
-                double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
-                double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
-                mom_add(red + 0, V.w[r], V.u[r], V.v[r], coeff * cx * shh * mh, coeff * cx * shb * mb);
-                mom_add(red + 4, V.w[r], V.u[r], V.v[r], coeff * cy * shh * mh, coeff * cy * shb * mb);
-            }
-            block_sum<8>(red, lds);
-            double tdx[4], tdy[4];
-            if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) { st = ST_SOLVE; break; }
-#pragma unroll
-            for (int r = 0; r < KPT; ++r) {
-                double qth[4], qtb[4];
-                qtheta(ck, th, V.u[r], V.v[r], qh[r], qth, qtb);
-                double mh = qh[r], mb = 0.5 * (double)ck * T * qh[r];
-                double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
-                double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
-                double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
-                double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
-                double dxh = 0, dxb = 0, dyh = 0, dyb = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    dxh += qth[q] * tdx[q]; dxb += qtb[q] * tdx[q];
-                    dyh += qth[q] * tdy[q]; dyb += qtb[q] * tdy[q];
-                }
-                amh[r] += wt2 * mh; amb[r] += wt2 * mb;
-                akxh[r] += wt2 * (kxh - dxh); akxb[r] += wt2 * (kxb - dxb);
-                akyh[r] += wt2 * (kyh - dyh); akyb[r] += wt2 * (kyb - dyb);
-            }
+    for (int n = 0; n < 9 && st == 0; ++n) {
+        const int cc = n / 3, a = n % 3;
+        double wq[4];
+        for (int m = 0; m < 4; ++m) wq[m] = nodes[m][a * 3 + cc] + W[m] - avg[m];
+        const double wt2 = KC.gw[a] * KC.gw[cc];
+        double th[4], J[16];
+        if (!closure_solve<NT, false>(ck, D, wq, th, J, nullptr, lds)) { st = ST_NEWTON; break; }
+        double rho, ux, vy, thg;
+        if (!s2_prim(ck, wq, &rho, &ux, &vy, &thg)) { st = ST_NONFINITE; break; }
+        const double lam = 0.5 / thg, coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho, T = th[3];
+        const double pre = th[0] / (UGKS_PI * T), iT = 1.0 / T;
+        double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int pos = 1;
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double mh = q_of(th, pre, iT, u, v), mb = 0.5 * (double)ck * T * mh;
+            pos &= (mh > 0.0 && isfinite(mh)) ? 1 : 0;
+            double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            mom_add(red + 0, w, u, v, coeff * cx * shh * mh, coeff * cx * shb * mb);
+            mom_add(red + 4, w, u, v, coeff * cy * shh * mh, coeff * cy * shb * mb);
+            amh[k] = (n == 0 ? 0.0 : amh[k]) + wt2 * mh;
+            amb[k] = (n == 0 ? 0.0 : amb[k]) + wt2 * mb;
         }
+        if (!__syncthreads_and(pos)) { st = ST_NEWTON; break; }
+        block_sum<8>(red, lds);
+        double tdx[4], tdy[4];
+        if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) { st = ST_SOLVE; break; }
+        if (threadIdx.x == 0) {
+            double* nd = node + n * CN_LEN;
+            for (int q = 0; q < 4; ++q) { nd[CN_TH + q] = th[q]; nd[CN_TDX + q] = tdx[q]; nd[CN_TDY + q] = tdy[q]; }
+            nd[CN_UX] = ux; nd[CN_VY] = vy; nd[CN_LAM] = lam; nd[CN_COEFF] = coeff; nd[CN_WT] = wt2; nd[CN_PRE] = pre;
+            nd[CN_IT] = iT; nd[CN_I0] = 1.0 / th[0];
+        }
+    }
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
     }
     // physical Shakhov collision of the current state
-    double hv[KPT], bv[KPT];
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        hv[r] = D.h[(size_t)c * nvel + k];
-        bv[r] = D.b[(size_t)c * nvel + k];
-    }
     double rho, ux, vy, thg;
     if (!s2_prim(ck, W, &rho, &ux, &vy, &thg)) {
-        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)ST_NONFINITE);
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_NONFINITE | ST_AT_CELL));
         return;
     }
+    const size_t cb0 = (size_t)c * nvel;
     double q[2] = {0.0, 0.0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        double cx = V.u[r] - ux, cy = V.v[r] - vy, c2 = cx * cx + cy * cy;
-        double dh = hv[r] - amh[r], db = bv[r] - amb[r];
-        q[0] += 0.5 * V.w[r] * cx * (c2 * dh + db);
-        q[1] += 0.5 * V.w[r] * cy * (c2 * dh + db);
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+        double dh = D.h[cb0 + k] - amh[k], db = D.b[cb0 + k] - amb[k];
+        q[0] += 0.5 * w * cx * (c2 * dh + db);
+        q[1] += 0.5 * w * cy * (c2 * dh + db);
     }
+    __syncthreads();   // node[] written by thread 0
     block_sum<2>(q, lds);
-    const double tau = tau_of_w(D.gas, W);
+    const double tau = tau_of_w(D.gas, W), itau = 1.0 / tau;
     const double dt = D.scal[SC_DT], inv_dt = 1.0 / dt;
     const double area = D.dx * D.dy, ainv = 1.0 / area;
     const int cw = j * nx + (i == 0 ? nx - 1 : i - 1);
     const size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    bool fin = true;
+    int fin = 1;
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double akxh = 0.0, akxb = 0.0, akyh = 0.0, akyb = 0.0;
+#pragma unroll 1
+        for (int n = 0; n < 9; ++n) {
+            const double* nd = node + n * CN_LEN;
+            const double th[4] = {nd[CN_TH], nd[CN_TH + 1], nd[CN_TH + 2], nd[CN_TH + 3]};
+            const double nux = nd[CN_UX], nvy = nd[CN_VY], lam = nd[CN_LAM], coeff = nd[CN_COEFF], wt2 = nd[CN_WT];
+            const double mh = q_of(th, nd[CN_PRE], nd[CN_IT], u, v), mb = 0.5 * (double)ck * th[3] * mh;
+            double qth[4], qtb[4];
+            qtheta(ck, th, nd[CN_I0], nd[CN_IT], u, v, mh, qth, qtb);
+            double cx = u - nux, cy = v - nvy, c2 = cx * cx + cy * cy;
+            double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
+            double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
+            double dxh = 0, dxb = 0, dyh = 0, dyb = 0;
 #pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        double eh = amh[r] + akxh[r] * q[0] + akyh[r] * q[1];
-        double eb = amb[r] + akxb[r] * q[0] + akyb[r] * q[1];
-        double ch = (eh - hv[r]) / tau, cb = (eb - bv[r]) / tau;
-        double sh = (D.xfl_h[(size_t)cw * nvel + k] * D.dy - D.xfl_h[(size_t)c * nvel + k] * D.dy +
+            for (int qq = 0; qq < 4; ++qq) {
+                dxh += qth[qq] * nd[CN_TDX + qq]; dxb += qtb[qq] * nd[CN_TDX + qq];
+                dyh += qth[qq] * nd[CN_TDY + qq]; dyb += qtb[qq] * nd[CN_TDY + qq];
+            }
+            akxh += wt2 * (kxh - dxh); akxb += wt2 * (kxb - dxb);
+            akyh += wt2 * (kyh - dyh); akyb += wt2 * (kyb - dyb);
+        }
+        const double hv = D.h[cb0 + k], bv = D.b[cb0 + k];
+        double eh = amh[k] + akxh * q[0] + akyh * q[1];
+        double eb = amb[k] + akxb * q[0] + akyb * q[1];
+        double ch = (eh - hv) * itau, cbv = (eb - bv) * itau;
+        double sh = (D.xfl_h[(size_t)cw * nvel + k] * D.dy - D.xfl_h[cb0 + k] * D.dy +
                      D.yfl_h[fs * nvel + k] * D.dx - D.yfl_h[fn * nvel + k] * D.dx) * ainv * inv_dt;
-        double sb = (D.xfl_b[(size_t)cw * nvel + k] * D.dy - D.xfl_b[(size_t)c * nvel + k] * D.dy +
+        double sb = (D.xfl_b[(size_t)cw * nvel + k] * D.dy - D.xfl_b[cb0 + k] * D.dy +
                      D.yfl_b[fs * nvel + k] * D.dx - D.yfl_b[fn * nvel + k] * D.dx) * ainv * inv_dt;
-        double th_ = sh + ch, tb_ = sb + cb;
-        D.rh[(size_t)c * nvel + k] = th_;
-        D.rb[(size_t)c * nvel + k] = tb_;
-        fin = fin && isfinite(th_) && isfinite(tb_);
-        red[0] += V.w[r] * th_ * th_;
-        red[1] += V.w[r] * tb_ * tb_;
-        red[2] += V.w[r] * fmax(hv[r] * hv[r], eh * eh);
-        red[3] += V.w[r] * fmax(bv[r] * bv[r], eb * eb);
-        mom_add(red + 4, V.w[r], V.u[r], V.v[r], ch, cb);
+        double th_ = sh + ch, tb_ = sb + cbv;
+        D.rh[cb0 + k] = th_;
+        D.rb[cb0 + k] = tb_;
+        fin &= (isfinite(th_) && isfinite(tb_)) ? 1 : 0;
+        red[0] += w * th_ * th_;
+        red[1] += w * tb_ * tb_;
+        red[2] += w * fmax(hv * hv, eh * eh);
+        red[3] += w * fmax(bv * bv, eb * eb);
+        mom_add(red + 4, w, u, v, ch, cbv);
     }
     block_sum<8>(red, lds);
-    if (!__syncthreads_and(fin ? 1 : 0)) st |= ST_NONFINITE;
+    if (!__syncthreads_and(fin)) st |= ST_NONFINITE;
     if (threadIdx.x == 0) {
         double rw[4], cm = 0.0, wm = 1.0;
         for (int m = 0; m < 4; ++m) {
@@ -882,7 +894,7 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
 // residual diagnostics (one workgroup): sums of the per-cell partials
 // ===========================================================================
 __global__ void __launch_bounds__(256) k_norms(Dev D, double residual_tol) {
-    __shared__ double lds[4 * 14];
+    __shared__ double lds[16 * 14];
     double s[14];
     for (int q = 0; q < 14; ++q) s[q] = 0.0;
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x)
@@ -936,26 +948,24 @@ __global__ void __launch_bounds__(256) k_norms(Dev D, double residual_tol) {
 }
 
 // ===========================================================================
-// cold start: every cell holds the discrete conservative equilibrium of W0
+// cold start: the rest state's discrete conservative equilibrium in every cell
 // ===========================================================================
-template <int KPT, int NT>
+template <int NT>
 __global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux, double tn) {
-    __shared__ double lds[(NT / 64) * 20];
-    VelR<KPT, NT> V;
-    load_vel<KPT, NT>(D, V);
+    __shared__ double lds[(NT / 16) * 20];
+    UGKS_DYN_SHARED(double, qh);
     const double lam = 1.0 / tn;
     double w[4] = {rho, rho * ux, 0.0, 0.5 * rho / (lam * (D.gas.gamma - 1.0)) + 0.5 * rho * (ux * ux)};
-    double th[4], J[16], qh[KPT];
-    if (!closure_solve<KPT, NT>(D.gas.ck, w, V, th, J, qh, lds)) {
+    double th[4], J[16];
+    if (!closure_solve<NT, true>(D.gas.ck, D, w, th, J, qh, lds)) {
         if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_NEWTON | ST_AT_COLD));
         return;
     }
+    const int nvel = D.nvel;
     for (int c = blockIdx.x; c < D.ncell; c += gridDim.x) {
-#pragma unroll
-        for (int r = 0; r < KPT; ++r) {
-            int k = threadIdx.x + NT * r;
-            D.h[(size_t)c * D.nvel + k] = qh[r];
-            D.b[(size_t)c * D.nvel + k] = 0.5 * (double)D.gas.ck * th[3] * qh[r];
+        UGKS_VLOOP(k, NT, nvel) {
+            D.h[(size_t)c * nvel + k] = qh[k];
+            D.b[(size_t)c * nvel + k] = 0.5 * (double)D.gas.ck * th[3] * qh[k];
         }
         if (threadIdx.x < 4) D.W[4 * c + threadIdx.x] = w[threadIdx.x];
     }
@@ -964,16 +974,14 @@ __global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux,
 // ===========================================================================
 // observables of the current state: rho, U, V, T, p, stresses, heat flux
 // ===========================================================================
-template <int KPT, int NT>
+template <int NT>
 __global__ void __launch_bounds__(NT) k_observables(Dev D, double* out) {
-    __shared__ double lds[(NT / 64) * 20];
+    __shared__ double lds[(NT / 16) * 20];
     const int c = blockIdx.x, nvel = D.nvel;
     double prim[4];
     primary(D.gas.gamma, D.W + 4 * c, prim);
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
+    UGKS_VLOOP(k, NT, nvel) {
         double h = D.h[(size_t)c * nvel + k], b = D.b[(size_t)c * nvel + k], w = D.w[k];
         double dvn = D.u[k] - prim[1], dvt = D.v[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
         s[0] += w * v2 * h;
@@ -994,57 +1002,77 @@ __global__ void __launch_bounds__(NT) k_observables(Dev D, double* out) {
     }
 }
 
-extern "C" hipError_t ugks_launch_cold_start(const Dev* Dp, double rho, double ux, double tn, hipStream_t s) {
-    const Dev& D = *Dp;
-    int nb = D.ncell < 1024 ? D.ncell : 1024;
-    switch (D.nvel) {
-    case 256: hipLaunchKernelGGL((k_cold_start<4, 64>), dim3(nb), dim3(64), 0, s, D, rho, ux, tn); break;
-    case 1024: hipLaunchKernelGGL((k_cold_start<4, 256>), dim3(nb), dim3(256), 0, s, D, rho, ux, tn); break;
-    case 4096: hipLaunchKernelGGL((k_cold_start<8, 512>), dim3(nb), dim3(512), 0, s, D, rho, ux, tn); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-extern "C" hipError_t ugks_launch_observables(const Dev* Dp, double* out, hipStream_t s) {
-    const Dev& D = *Dp;
-    switch (D.nvel) {
-    case 256: hipLaunchKernelGGL((k_observables<4, 64>), dim3(D.ncell), dim3(64), 0, s, D, out); break;
-    case 1024: hipLaunchKernelGGL((k_observables<4, 256>), dim3(D.ncell), dim3(256), 0, s, D, out); break;
-    case 4096: hipLaunchKernelGGL((k_observables<8, 512>), dim3(D.ncell), dim3(512), 0, s, D, out); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 // ===========================================================================
 // launchers
 // ===========================================================================
 extern "C" hipError_t ugks_set_consts(const Consts* k) { return hipMemcpyToSymbol(HIP_SYMBOL(KC), k, sizeof(Consts)); }
 
-template <int KPT, int NT>
-static void launch_closure_kernels(const Dev& D, hipStream_t s) {
-    hipLaunchKernelGGL((k_face_flux<KPT, NT>), dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL((k_wall_flux<KPT, NT>), dim3(2 * D.nx), dim3(NT), 0, s, D);
+// Allow the dynamic LDS a kernel needs beyond the default limit (once per kernel).
+template <typename K>
+static hipError_t allow_lds(K kernel, size_t bytes, size_t* done) {
+    if (bytes <= *done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes);
+    if (e == hipSuccess) *done = bytes;
+    return e;
 }
 
-template <int KPT, int NT>
-static void launch_cell_kernel(const Dev& D, hipStream_t s) {
-    hipLaunchKernelGGL((k_cell<KPT, NT>), dim3(D.ncell), dim3(NT), 0, s, D);
+template <int NT>
+static hipError_t launch_cold_start(const Dev& D, double rho, double ux, double tn, hipStream_t s) {
+    static size_t lds_cold = 0;
+    const size_t q1 = sizeof(double) * D.nvel;
+    hipError_t e = allow_lds(k_cold_start<NT>, q1, &lds_cold);
+    if (e != hipSuccess) return e;
+    int nb = D.ncell < 1024 ? D.ncell : 1024;
+    hipLaunchKernelGGL(k_cold_start<NT>, dim3(nb), dim3(NT), q1, s, D, rho, ux, tn);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ugks_launch_cold_start(const Dev* Dp, double rho, double ux, double tn, hipStream_t s) {
+    switch (Dp->wg_vel) {
+    case 64: return launch_cold_start<64>(*Dp, rho, ux, tn, s);
+    case 128: return launch_cold_start<128>(*Dp, rho, ux, tn, s);
+    case 256: return launch_cold_start<256>(*Dp, rho, ux, tn, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t ugks_launch_observables(const Dev* Dp, double* out, hipStream_t s) {
+    const Dev& D = *Dp;
+    hipLaunchKernelGGL(k_observables<256>, dim3(D.ncell), dim3(256), 0, s, D, out);
+    return hipGetLastError();
+}
+
+// face, wall and cell kernels with NT-thread workgroups
+template <int NT>
+static hipError_t launch_closure_kernels(const Dev& D, hipStream_t s, Prof* P) {
+    static size_t lds_face = 0, lds_wall = 0, lds_cell = 0;
+    const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
+    hipError_t e;
+    if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
+    if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
+    if ((e = allow_lds(k_cell<NT>, q2, &lds_cell)) != hipSuccess) return e;
+    UGKS_TIMED(P, K_FACE_FLUX, s,
+               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), q1, s, D));
+    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx), dim3(NT), q1, s, D));
+    UGKS_TIMED(P, K_CELL, s, hipLaunchKernelGGL(k_cell<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
+    return hipSuccess;
 }
 
 extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vmax, double residual_tol,
-                                           hipStream_t s) {
+                                           hipStream_t s, Prof* P) {
     const Dev& D = *Dp;
-    hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax);
-    hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_recon, dim3((D.nvel + 127) / 128, D.ncell), dim3(128), 0, s, D);
-    switch (D.nvel) {
-    case 256: launch_closure_kernels<4, 64>(D, s); launch_cell_kernel<4, 64>(D, s); break;
-    case 1024: launch_closure_kernels<4, 256>(D, s); launch_cell_kernel<4, 256>(D, s); break;
-    case 4096: launch_closure_kernels<8, 512>(D, s); launch_cell_kernel<8, 512>(D, s); break;
-    default: return hipErrorInvalidValue;
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
+    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3((D.nvel + 127) / 128, D.ncell), dim3(128), 0, s, D));
+    hipError_t e;
+    switch (D.wg_vel) {
+    case 64: e = launch_closure_kernels<64>(D, s, P); break;
+    case 128: e = launch_closure_kernels<128>(D, s, P); break;
+    case 256: e = launch_closure_kernels<256>(D, s, P); break;
+    default: e = hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, s, D, residual_tol);
+    if (e != hipSuccess) return e;
+    UGKS_TIMED(P, K_NORMS, s, hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, s, D, residual_tol));
     return hipGetLastError();
 }

@@ -1,0 +1,81 @@
+// Per-kernel timing with HIP events recorded on the launch stream.
+//
+// When enabled, every launch is bracketed by two events from a pool; the
+// elapsed times are collected when the host reads them (after the stream has
+// drained), so recording adds no host synchronisation to the step.
+#pragma once
+#include <vector>
+
+#ifndef UGKS_EMULATE
+#include <hip/hip_runtime.h>
+#endif
+
+enum KernelId {
+    K_TIMESTEP, K_WRECON, K_RECON, K_FACE_FLUX, K_WALL_FLUX, K_CELL, K_NORMS,
+    K_MACRO_GAMMA, K_MACRO_PR2, K_PREDICT, K_SWEEP_FWD, K_SWEEP_BWD, K_RECOVER, K_MASS, K_RESCALE,
+    K_MASS_AFTER, K_COLD_START, K_OBSERVABLES, K_WALL_REFLECTION, K_COUNT
+};
+
+inline const char* kernel_name(int id) {
+    static const char* names[K_COUNT] = {
+        "k_timestep", "k_wrecon", "k_recon", "k_face_flux", "k_wall_flux", "k_cell", "k_norms",
+        "k_macro_gamma", "k_macro_pr2", "k_predict", "k_sweep_fwd", "k_sweep_bwd", "k_recover", "k_mass",
+        "k_rescale", "k_mass_after", "k_cold_start", "k_observables", "k_wall_reflection"};
+    return (id >= 0 && id < K_COUNT) ? names[id] : "";
+}
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Rec { int id; size_t e0, e1; };
+    std::vector<Rec> recs;
+    double total_ms[K_COUNT] = {};
+    long launches[K_COUNT] = {};
+
+    size_t mark(hipStream_t s) {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+            pool.push_back(e);
+        }
+        hipEventRecord(pool[used], s);
+        return used++;
+    }
+    // fold recorded pairs into the totals; the stream must have drained
+    void collect() {
+        for (const Rec& r : recs) {
+            float ms = 0.f;
+            if (r.e0 != (size_t)-1 && r.e1 != (size_t)-1 && hipEventElapsedTime(&ms, pool[r.e0], pool[r.e1]) == hipSuccess) {
+                total_ms[r.id] += ms;
+                launches[r.id] += 1;
+            }
+        }
+        recs.clear();
+        used = 0;
+    }
+    void reset() {
+        recs.clear();
+        used = 0;
+        for (int k = 0; k < K_COUNT; ++k) { total_ms[k] = 0.0; launches[k] = 0; }
+    }
+    void release() {
+        for (hipEvent_t e : pool) hipEventDestroy(e);
+        pool.clear();
+        used = 0;
+        recs.clear();
+    }
+};
+
+// Launch `...` (a hipLaunchKernelGGL statement) bracketed by events when P is on.
+#define UGKS_TIMED(P, id, s, ...)                                        \
+    do {                                                                 \
+        if ((P) && (P)->on) {                                            \
+            size_t e0_ = (P)->mark(s);                                   \
+            __VA_ARGS__;                                                 \
+            size_t e1_ = (P)->mark(s);                                   \
+            (P)->recs.push_back(Prof::Rec{(int)(id), e0_, e1_});         \
+        } else {                                                         \
+            __VA_ARGS__;                                                 \
+        }                                                                \
+    } while (0)

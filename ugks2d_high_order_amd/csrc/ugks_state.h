@@ -27,6 +27,7 @@ struct Dev {
     double wall_ux[2], wall_tn[2];
     int macro_sweeps;
     int tile;             // velocities per sweep workgroup
+    int wg_vel;           // workgroup size of the velocity-parallel operator kernels (64, 128 or 256)
     int ntiles;
     // velocity table
     const double *u, *v, *w;
