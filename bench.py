@@ -35,11 +35,19 @@ sys.path.insert(0, ROOT)
 # (8): 18 FP64 cv-arrays.
 HOI_BYTES_PER_CV = 18 * 8
 
-# FP64 operations per face-velocity of k_face_flux (a face-velocity is one
-# velocity of one of the ncell + nx*(ny-1) interior faces): ADD + MUL + 2*FMA
-# + TRANS F64 VALU instructions x 64 lanes from a rocprofv3 --pmc pass over
-# this workload, divided by the face-velocities of the launch.
-FACE_FLUX_FLOP_PER_FACE_VELOCITY = None
+# FP64 operations of the operator kernels per unit of work, from rocprofv3
+# --pmc passes over this workload (ADD + MUL + 2 FMA + TRANS F64 VALU
+# instructions x 64 lanes per launch, divided by the launch's units), and HBM
+# bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB -> bytes; the
+# factors make the counters read a 1 GiB device copy as 1 GiB each way).
+# Units: "cv" = cell-velocities nx*ny*Nvel, "fv" = interior face-velocities
+# (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
+KERNEL_COUNTS = {
+    # kernel: (flop per unit, unit, HBM bytes per launch)
+    "k_recon": (None, "cv", None),
+    "k_face_flux": (None, "fv", None),
+    "k_cell": (None, "cv", None),
+}
 
 MI355X_HBM_GBPS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
@@ -48,7 +56,7 @@ MI355X_FP64_TFLOPS = 78.6
 def _args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["large", "shipped"], default="large")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -158,16 +166,17 @@ def main():
     name = max(kt, key=lambda k: kt[k][0])
     ms, launches = kt[name]
     avg_s = ms / max(launches, 1) / 1e3
-    if name == "k_face_flux" and FACE_FLUX_FLOP_PER_FACE_VELOCITY is not None:
-        units = (case.nx * case.ny + case.nx * (case.ny - 1)) * case.nu * case.nv
-        ach = FACE_FLUX_FLOP_PER_FACE_VELOCITY * units / avg_s / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 valu (no dense contraction, no MFMA)", "kernel": name,
+    flop, unit, traffic = KERNEL_COUNTS.get(name, (None, None, None))
+    units = {"cv": cv, "fv": (case.nx * case.ny + case.nx * (case.ny - 1)) * case.nu * case.nv}.get(unit)
+    if flop is not None and units:
+        ach = flop * units / avg_s / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64 valu (no dense contraction, so no MFMA)", "kernel": name,
                 "achieved": ach, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_TFLOPS,
-                "traffic": None, "avg_launch_ms": avg_s * 1e3}
+                "traffic": traffic, "flop_per_unit": flop, "work_unit": unit, "avg_launch_ms": avg_s * 1e3}
     else:
-        ach = HOI_BYTES_PER_CV * cv / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
-                "frac": ach / MI355X_HBM_GBPS, "traffic": None, "avg_launch_ms": avg_s * 1e3}
+        roof = {"bound": "mfma", "pipe": "fp64 valu", "kernel": name, "achieved": None,
+                "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": None, "traffic": traffic,
+                "avg_launch_ms": avg_s * 1e3}
     kernels = {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0]) if v[1]}
 
     cpu = None

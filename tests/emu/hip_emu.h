@@ -139,6 +139,7 @@ inline void emu_launch(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args
 
 // workgroup reductions: one exchange instead of lane butterflies
 #define UGKS_EMU_REDUCTIONS 1
+inline double lane_xor1(double x) { return __shfl_xor(x, 1, 64); }
 template <int R>
 inline void block_sum(double* v, double*) {
     EmuBlock& B = *emu_block;

@@ -194,7 +194,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     D.wall_tn[0] = p->wall_tn[0]; D.wall_tn[1] = p->wall_tn[1];
     D.macro_sweeps = p->macro_sweeps;
     D.tile = 16;
-    D.wg_vel = 64;
+    D.wg_vel = 128;
     D.ntiles = nvel / D.tile;
     I->umax = -1e300; I->vmax = -1e300;
     for (int k = 0; k < nvel; ++k) { I->umax = std::max(I->umax, u[k]); I->vmax = std::max(I->vmax, v[k]); }
@@ -210,8 +210,8 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     for (auto a : cvs) if ((rc = dmalloc(I, a, cv))) return bail(rc);
     if ((rc = dmalloc(I, &D.yfl_h, (ny + 1) * nx * nvel)) || (rc = dmalloc(I, &D.yfl_b, (ny + 1) * nx * nvel)) ||
         (rc = dmalloc(I, &D.xfw, 4 * nc)) || (rc = dmalloc(I, &D.yfw, 4 * (ny + 1) * nx)) ||
-        (rc = dmalloc(I, &D.xf, nc * 18 * nvel)) || (rc = dmalloc(I, &D.yf, nx * (ny - 1) * 18 * nvel)) ||
-        (rc = dmalloc(I, &D.wb, nx * 18 * nvel)) || (rc = dmalloc(I, &D.wt, nx * 18 * nvel)) ||
+        (rc = dmalloc(I, &D.xf, nc * REC_ROWS * nvel)) || (rc = dmalloc(I, &D.yf, nx * (ny - 1) * REC_ROWS * nvel)) ||
+        (rc = dmalloc(I, &D.wb, nx * REC_ROWS * nvel)) || (rc = dmalloc(I, &D.wt, nx * REC_ROWS * nvel)) ||
         (rc = dmalloc(I, &D.ws, nc * 4 * 24)) || (rc = dmalloc(I, &D.wv, nc * 4 * 9)) ||
         (rc = dmalloc(I, &D.cellpart, nc * 16)) || (rc = dmalloc(I, &D.gx, nc)) ||
         (rc = dmalloc(I, &D.gy, (ny + 1) * nx)) || (rc = dmalloc(I, &D.tau_t, nc)) ||
@@ -384,8 +384,8 @@ extern "C" int ugks_debug_fetch(void* handle, int which, double* out, long n) {
     const Dev& D = I->D;
     const size_t nv = D.nvel, nc = D.ncell;
     const double* src[] = {D.xf, D.yf, D.wb, D.wt, D.ws, D.wv, D.scal};
-    const size_t len[] = {nc * 18 * nv, (size_t)D.nx * (D.ny - 1) * 18 * nv, (size_t)D.nx * 18 * nv,
-                          (size_t)D.nx * 18 * nv, nc * 96, nc * 36, SC_COUNT};
+    const size_t len[] = {nc * REC_ROWS * nv, (size_t)D.nx * (D.ny - 1) * REC_ROWS * nv, (size_t)D.nx * REC_ROWS * nv,
+                          (size_t)D.nx * REC_ROWS * nv, nc * 96, nc * 36, SC_COUNT};
     if (which < 0 || which > 6) return fail(E_ARG, "unknown buffer");
     size_t m = std::min((size_t)n, len[which]);
     HIPCHK(hipMemcpyAsync(out, src[which], sizeof(double) * m, hipMemcpyDeviceToHost, I->stream));

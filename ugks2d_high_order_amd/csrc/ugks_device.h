@@ -82,6 +82,9 @@ __device__ inline double row16_max(double x) {
     x = fmax(x, dpp_f64<DPP_MIRROR>(x));
     return x;
 }
+// value of the partner lane (lane xor 1)
+__device__ inline double lane_xor1(double x) { return dpp_f64<DPP_XOR1>(x); }
+
 // wavefront (64 lanes) totals, identical in every lane
 __device__ inline double wave_sum(double x) {
     x = row16_sum(x);
@@ -266,9 +269,9 @@ __device__ inline void weno_weights(const double q[5], double fw[4]) {
     double tau = (fabs(b3 - b0) + fabs(b3 - b1) + fabs(b3 - b2)) / 3.0;
     double r0 = tau / (b0 + 1.0e-8), r1 = tau / (b1 + 1.0e-8), r2 = tau / (b2 + 1.0e-8), r3 = tau / (b3 + 1.0e-8);
     double a0 = g0 * (1.0 + r0 * r0), a1 = g1 * (1.0 + r1 * r1), a2 = g2 * (1.0 + r2 * r2), a3 = g3 * (1.0 + r3 * r3);
-    double sa = a0 + a1 + a2 + a3;
-    double o0 = a0 / sa, o1 = a1 / sa, o2 = a2 / sa, o3 = a3 / sa;
-    fw[3] = o3 / g3;
+    double isa = 1.0 / (a0 + a1 + a2 + a3);
+    double o0 = a0 * isa, o1 = a1 * isa, o2 = a2 * isa, o3 = a3 * isa;
+    fw[3] = o3 * (1.0 / g3);
     fw[0] = o0 - fw[3] * g0;
     fw[1] = o1 - fw[3] * g1;
     fw[2] = o2 - fw[3] * g2;

@@ -126,26 +126,27 @@ __device__ inline double theta_of(double m, double fval[4][3], double vol[3][3],
     return theta;
 }
 
-// write one limited side for h (q 0..2) and b (q 3..5) into a face record
-__device__ inline void store_side(double* rec, int nvel, int k, double theta, double hm, double bm,
-                                  const SideOut& sh_, const SideOut& sb_, double sn, double st) {
+// write one limited side of one field (rows off..off+2 of each node) into a
+// face record; isn, ist: reciprocals of the normal and tangential cell sizes
+__device__ inline void store_side(double* rec, size_t nv, int off, double theta, double m, const SideOut& so,
+                                  double isn, double ist) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double* p = rec + (size_t)(a * 6) * nvel + k;
-        p[0 * (size_t)nvel] = hm + theta * (sh_.d[a][0] - hm);
-        p[1 * (size_t)nvel] = theta * sh_.d[a][1] / sn;
-        p[2 * (size_t)nvel] = theta * sh_.d[a][2] / st;
-        p[3 * (size_t)nvel] = bm + theta * (sb_.d[a][0] - bm);
-        p[4 * (size_t)nvel] = theta * sb_.d[a][1] / sn;
-        p[5 * (size_t)nvel] = theta * sb_.d[a][2] / st;
+        double* p = rec + (size_t)(a * 6 + off) * nv;
+        p[0] = m + theta * (so.d[a][0] - m);
+        p[nv] = theta * so.d[a][1] * isn;
+        p[2 * nv] = theta * so.d[a][2] * ist;
     }
 }
 
+// Two lanes per (cell, velocity): the even lane reconstructs h, the odd lane
+// b; the positivity factor is the minimum of the pair, exchanged through DPP.
 __global__ void __launch_bounds__(128) k_recon(Dev D) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t >> 1, fld = t & 1;
     const int c = blockIdx.y;
-    if (k >= D.nvel) return;
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    const size_t nv = (size_t)nvel;
     const int i = c % nx, j = c / nx;
     int rb, sh;
     bool near;
@@ -154,29 +155,29 @@ __global__ void __launch_bounds__(128) k_recon(Dev D) {
     const int xs = uk >= 0.0 ? 1 : 0;
     const int ys = vk >= 0.0 ? 3 : 2;
     const int wsd = j == 0 ? 2 : (j == ny - 1 ? 3 : -1);
+    const double* F = fld == 0 ? D.h : D.b;
     double fv[4][3], vol[3][3];
-    SideOut hx, hy, hw, bx, by, bw;
-    const double hm = D.h[(size_t)c * nvel + k], bm = D.b[(size_t)c * nvel + k];
-    double theta = 1.0;
-    recon_field(D.h, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, hx, hy, hw);
-    theta = theta_of(hm, fv, vol, theta);
-    recon_field(D.b, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, bx, by, bw);
-    theta = theta_of(bm, fv, vol, theta);
-    if (hm <= 0.0 || bm <= 0.0) theta = 0.0;
+    SideOut xo, yo, wo;
+    const double m = F[(size_t)c * nv + k];
+    recon_field(F, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, xo, yo, wo);
+    double theta = theta_of(m, fv, vol, 1.0);
+    theta = fmin(theta, lane_xor1(theta));
+    const double m_other = lane_xor1(m);
+    if (m <= 0.0 || m_other <= 0.0) theta = 0.0;
     theta = fmax(0.0, fmin(1.0, theta));
+    const double idx = 1.0 / D.dx, idy = 1.0 / D.dy;
+    const int off = 3 * fld;
     // x face: upwind side of this cell
     {
         int f = xs == 1 ? c : j * nx + (i == 0 ? nx - 1 : i - 1);
-        store_side(D.xf + (size_t)f * 18 * nvel, nvel, k, theta, hm, bm, hx, bx, D.dx, D.dy);
+        store_side(D.xf + (size_t)f * REC_ROWS * nv + k, nv, off, theta, m, xo, idx, idy);
     }
     // interior y faces
-    if (ys == 3 && j < ny - 1)
-        store_side(D.yf + (size_t)c * 18 * nvel, nvel, k, theta, hm, bm, hy, by, D.dy, D.dx);
-    if (ys == 2 && j > 0)
-        store_side(D.yf + (size_t)(c - nx) * 18 * nvel, nvel, k, theta, hm, bm, hy, by, D.dy, D.dx);
+    if (ys == 3 && j < ny - 1) store_side(D.yf + (size_t)c * REC_ROWS * nv + k, nv, off, theta, m, yo, idy, idx);
+    if (ys == 2 && j > 0) store_side(D.yf + (size_t)(c - nx) * REC_ROWS * nv + k, nv, off, theta, m, yo, idy, idx);
     // wall rows: the wall side at every velocity
-    if (j == 0) store_side(D.wb + (size_t)i * 18 * nvel, nvel, k, theta, hm, bm, hw, bw, D.dy, D.dx);
-    if (j == ny - 1) store_side(D.wt + (size_t)i * 18 * nvel, nvel, k, theta, hm, bm, hw, bw, D.dy, D.dx);
+    if (j == 0) store_side(D.wb + (size_t)i * REC_ROWS * nv + k, nv, off, theta, m, wo, idy, idx);
+    if (j == ny - 1) store_side(D.wt + (size_t)i * REC_ROWS * nv + k, nv, off, theta, m, wo, idy, idx);
 }
 
 // ===========================================================================
@@ -475,14 +476,14 @@ __global__ void __launch_bounds__(NT) k_face_flux(Dev D) {
         int j = f / nx, i = f % nx;
         cl = f;
         cr = j * nx + (i == nx - 1 ? 0 : i + 1);
-        rec = D.xf + (size_t)f * 18 * nvel;
+        rec = D.xf + (size_t)f * REC_ROWS * nvel;
         oh = D.xfl_h + (size_t)f * nvel; ob = D.xfl_b + (size_t)f * nvel; ow = D.xfw + 4 * (size_t)f;
     } else {
         axis = 2;
         int g = f - nfx;   // face between rows j and j+1, g = j*nx+i
         cl = g;
         cr = g + nx;
-        rec = D.yf + (size_t)g * 18 * nvel;
+        rec = D.yf + (size_t)g * REC_ROWS * nvel;
         size_t fo = (size_t)g + nx;   // south face of row j+1
         oh = D.yfl_h + fo * nvel; ob = D.yfl_b + fo * nvel; ow = D.yfw + 4 * fo;
     }
@@ -622,7 +623,7 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
     const int top = blockIdx.x >= nx ? 1 : 0, i = blockIdx.x % nx;
     const int side = top ? 1 : -1;
     const int c = top ? (ny - 1) * nx + i : i;
-    const double* rec = (top ? D.wt : D.wb) + (size_t)i * 18 * nvel;
+    const double* rec = (top ? D.wt : D.wb) + (size_t)i * REC_ROWS * nvel;
     const size_t fo = (size_t)(top ? ny : 0) * nx + i;
     double* oh = D.yfl_h + fo * nv;
     double* ob = D.yfl_b + fo * nv;
@@ -1064,7 +1065,7 @@ extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vm
     const Dev& D = *Dp;
     UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
     UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D));
-    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3((D.nvel + 127) / 128, D.ncell), dim3(128), 0, s, D));
+    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell), dim3(128), 0, s, D));
     hipError_t e;
     switch (D.wg_vel) {
     case 64: e = launch_closure_kernels<64>(D, s, P); break;

@@ -20,6 +20,11 @@ enum ScalarSlot {
     SC_COUNT = 64
 };
 
+// Upwind face record, rows of nvel values: for each Gauss node a (rows
+// 6a..6a+5) the limited h value, h normal and tangential derivatives (per
+// unit length), b value, b derivatives.
+enum { REC_ROWS = 18 };
+
 struct Dev {
     int nx, ny, nu, nv, nvel, ncell;
     double dx, dy, cfl, ncfl;
