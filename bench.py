@@ -43,10 +43,10 @@ HOI_BYTES_PER_CV = 18 * 8
 # Units: "cv" = cell-velocities nx*ny*Nvel, "fv" = interior face-velocities
 # (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
 KERNEL_COUNTS = {
-    # kernel: (flop per unit, unit, HBM bytes per launch)
-    "k_recon": (None, "cv", None),
-    "k_face_flux": (None, "fv", None),
-    "k_cell": (None, "cv", None),
+    # kernel: (flop per unit, unit, HBM bytes per launch); 64x512 cells x 64x64 velocities
+    "k_recon": (16250.1, "cv", 4.9571e10),
+    "k_face_flux": (1827.4, "fv", 8.8191e10),
+    "k_cell": (3065.3, "cv", 1.5597e10),
 }
 
 MI355X_HBM_GBPS = 8000.0

@@ -12,7 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libugks_hip.so")
 SOURCES = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_api.hip"]
-HEADERS = ["ugks_device.h", "ugks_recon.h", "ugks_state.h"]
+HEADERS = ["ugks_device.h", "ugks_recon.h", "ugks_state.h", "ugks_prof.h"]
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "couette_driver")
 ARCH = os.environ.get("UGKS_OFFLOAD_ARCH", "gfx950")
 
 
@@ -73,6 +75,36 @@ def build(force: bool = True, verbose: bool = False) -> str:
     return LIB
 
 
+def flang() -> str | None:
+    for c in ("/opt/rocm/lib/llvm/bin/amdflang", "/opt/rocm/llvm/bin/amdflang"):
+        if os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the Fortran binding module and the Couette driver against
+    libugks_hip.so (skipped when no Fortran compiler is installed)."""
+    fc = flang()
+    if fc is None:
+        return None
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    mod_o = os.path.join(bdir, "ugks_hip_binding.o")
+    cmds = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN, "ugks_hip_binding.f90"), "-o", mod_o, "-module-dir", bdir],
+        [fc, "-O2", "-I", bdir, os.path.join(FORTRAN, "couette_driver.f90"), mod_o, "-L", HERE, "-lugks_hip",
+         "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build(force=True, verbose="-v" in sys.argv)
     print(LIB)
+    d = build_fortran(verbose="-v" in sys.argv)
+    print(d or "no Fortran compiler: driver not built")

@@ -1,0 +1,112 @@
+! Fortran binding of the C ABI in include/ugks_hip.h.
+!
+! A host built on the reference's Fortran modules calls the GPU step through
+! these interfaces: the derived type mirrors ugks_params_t field by field, and
+! every procedure is the bind(C) image of one ABI function.  Arrays are the
+! reference's own column-major stores, W(4,nx,ny) and h(Nvel,nx,ny).
+module ugks_hip
+  use iso_c_binding
+  implicit none
+
+  type, bind(C) :: ugks_params_t
+    integer(c_int) :: nx, ny, nu, nv, ck
+    integer(c_int) :: macro_sweeps, micro_sweeps, mass_rescale
+    real(c_double) :: dx, dy, cfl, ncfl, pr, omega, mu_ref
+    real(c_double) :: wall_ux(2), wall_tn(2)
+    real(c_double) :: min_alpha, moment_tol, residual_tol
+  end type ugks_params_t
+
+  interface
+    integer(c_int) function ugks_create(p, u, v, w, handle) bind(C, name="ugks_create")
+      import :: c_int, c_double, c_ptr, ugks_params_t
+      type(ugks_params_t), intent(in) :: p
+      real(c_double), intent(in) :: u(*), v(*), w(*)
+      type(c_ptr), intent(out) :: handle
+    end function ugks_create
+
+    subroutine ugks_destroy(handle) bind(C, name="ugks_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: handle
+    end subroutine ugks_destroy
+
+    type(c_ptr) function ugks_last_error() bind(C, name="ugks_last_error")
+      import :: c_ptr
+    end function ugks_last_error
+
+    integer(c_int) function ugks_set_state(handle, W, h, b) bind(C, name="ugks_set_state")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: W(*), h(*), b(*)
+    end function ugks_set_state
+
+    integer(c_int) function ugks_get_state(handle, W, h, b) bind(C, name="ugks_get_state")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: W(*), h(*), b(*)
+    end function ugks_get_state
+
+    integer(c_int) function ugks_cold_start(handle, rho, ux, t_native) bind(C, name="ugks_cold_start")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), value :: rho, ux, t_native
+    end function ugks_cold_start
+
+    integer(c_int) function ugks_evolution(handle, dt, rate_w, rate_h, rate_b, xflux_h, xflux_b, yflux_h, yflux_b) &
+        bind(C, name="ugks_evolution")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      type(c_ptr), value :: dt, rate_w, rate_h, rate_b, xflux_h, xflux_b, yflux_h, yflux_b
+    end function ugks_evolution
+
+    integer(c_int) function ugks_norms(handle, out24, walls16) bind(C, name="ugks_norms")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: out24(24), walls16(16)
+    end function ugks_norms
+
+    integer(c_int) function ugks_implicit_step(handle, accepted, diag16) bind(C, name="ugks_implicit_step")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), intent(out) :: accepted
+      real(c_double), intent(out) :: diag16(16)
+    end function ugks_implicit_step
+
+    integer(c_int) function ugks_run(handle, n, done) bind(C, name="ugks_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: n
+      integer(c_int), intent(out) :: done
+    end function ugks_run
+
+    integer(c_int) function ugks_observables(handle, fields10) bind(C, name="ugks_observables")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: fields10(*)
+    end function ugks_observables
+
+    integer(c_int) function ugks_sync(handle) bind(C, name="ugks_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function ugks_sync
+  end interface
+
+contains
+
+  ! message of the calling thread's last failure
+  function ugks_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: n
+    p = ugks_last_error()
+    call c_f_pointer(p, s, [4096])
+    n = 0
+    do while (n < 4096)
+      if (s(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function ugks_error_text
+
+end module ugks_hip
