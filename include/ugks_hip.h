@@ -89,6 +89,24 @@ int ugks_norms(void* handle, double* out24, double* walls16);
  * moment_error, M_before, M_after, M0, dt, dtp, status bits, ...]. */
 int ugks_implicit_step(void* handle, int* accepted, double* diag16);
 
+/* One S2CN2 explicit step (the TimeIntegration_method = 'S2CN2' path):
+ * the S2 fluxes of the current state (as in ugks_evolution), the streamed
+ * state, volume targets at the streamed macro state and the Crank-Nicolson
+ * Shakhov collision with variable tau, then the CN and moment gates and the
+ * global conservation gate with the wall ledger.  Replaces S2NativeEvolution
+ * + S2NativeApplyUpdate (src/mod_S2NativeAdapter.f90:469-599,867-1018;
+ * advance_step, src/mod_S2D1.f90:1302-1435).  On a gate rejection the state
+ * is unchanged, *accepted = 0 and 0 is returned.  diag16: [1] target error
+ * (0), [2] CN residual, [3] moment error, [6] max 2 beta, [7] CFL number,
+ * [8..11] min/max of h and b, [12] conservation drift, [13] dt,
+ * [14] 1 if rejected by the CN/moment gate, [15] 1 if by the drift gate.
+ * The targets of the state and the ledger carry over between calls; a new
+ * state (ugks_set_state, ugks_cold_start) starts a new ledger. */
+int ugks_cn_step(void* handle, int* accepted, double* diag16);
+/* S2CN2 ledger: conserved totals at the start of the run (4), cumulative wall
+ * balance (4) (the reference's total0 and wall_balance). */
+int ugks_cn_ledger(void* handle, double* out8);
+
 /* Run n iterations (operator + update each), stopping early on rejection.
  * Returns the number of accepted iterations in *done. */
 int ugks_run(void* handle, int n, int* done);

@@ -60,6 +60,9 @@ def lib():
         L.or_get_operator.argtypes = [C.c_void_p] + [_D] * 12
         L.or_get_norms.argtypes = [C.c_void_p, _D]
         L.or_implicit_update.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+        L.or_cn_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+        L.or_cn_ledger.argtypes = [C.c_void_p, _D]
+        L.or_cn_ledger.restype = None
         L.or_iteration.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
         L.or_observables.argtypes = [C.c_void_p, _D, _D]
         L.or_get_implicit.argtypes = [C.c_void_p, _D, _D, _D, _D]
@@ -167,6 +170,18 @@ class OracleSolver:
         d = np.zeros(16)
         st = lib().or_iteration(self.h, C.byref(acc), ptr(d))
         return st, bool(acc.value), d
+
+    def cn_step(self):
+        """One S2CN2 explicit step; returns (status, accepted, diag16)."""
+        acc = C.c_int(0)
+        d = np.zeros(16)
+        st = lib().or_cn_step(self.h, C.byref(acc), ptr(d))
+        return st, bool(acc.value), d
+
+    def cn_ledger(self):
+        o = np.zeros(8)
+        lib().or_cn_ledger(self.h, ptr(o))
+        return o[:4], o[4:]
 
     def implicit_fields(self):
         nc, n = self.ncell, self.nvel

@@ -60,6 +60,8 @@ struct or_solver {
     double M0;
     int have_M0;
     double last_diag[16];
+    int cn_ledger;            /* S2CN2 conservation ledger initialised */
+    double cn_total0[4], cn_balance[4];
 };
 
 static inline int cidx(const or_solver *s, int i, int j) { return j * s->p.nx + i; }
@@ -1117,6 +1119,7 @@ int or_cold_start(or_solver *s, double rho, double ux, double tnative) {
     free(tmp);
     closure_ws_free(&c);
     s->have_M0 = 0;
+    s->cn_ledger = 0;
     s->ref_init = 0;
     return ok ? OR_OK : OR_NEWTON;
 }
@@ -1126,6 +1129,7 @@ int or_set_state(or_solver *s, const double *W, const double *h, const double *b
     memcpy(s->h, h, sizeof(double) * s->ncell * s->nvel);
     memcpy(s->b, b, sizeof(double) * s->ncell * s->nvel);
     s->have_M0 = 0;
+    s->cn_ledger = 0;
     return OR_OK;
 }
 
@@ -1297,15 +1301,17 @@ static int compute_flux(or_solver *s) {
 }
 
 /* build_targets: 3x3 volume nodes of the WENO tensor W, mean-corrected, each
-   closed by make_mke and averaged with the Gauss weights,
-   src/mod_S2D1.f90:1246-1300 */
-static int build_targets(or_solver *s) {
+   closed by make_mke and averaged with the Gauss weights; kappa (optional,
+   [cell][4] = heat(W, kx) then heat(W, ky)) and the largest make_mke moment
+   residual (optional), src/mod_S2D1.f90:1246-1300 */
+static int build_targets_of(or_solver *s, const double *Wsrc, double *omh, double *omb, double *okxh, double *okxb,
+                            double *okyh, double *okyb, double *okappa, double *target_err) {
     const or_params *P = &s->p;
     int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell;
     double *nodes = dalloc((size_t)nc * 36);   /* [cell][m][a][c] */
     double *Fm = dalloc(nc);
     for (int m = 0; m < 4; ++m) {
-        for (int c = 0; c < nc; ++c) Fm[c] = s->W[4 * c + m];
+        for (int c = 0; c < nc; ++c) Fm[c] = Wsrc[4 * c + m];
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 double f[4][3], dn[4][3], dt[4][3], vol[3][3];
@@ -1316,21 +1322,23 @@ static int build_targets(or_solver *s) {
     }
     free(Fm);
     int status = OR_OK;
+    double terr = 0.0;
 #pragma omp parallel
     {
         closure_ws c;
         closure_ws_init(&c, n);
         double *t = dalloc((size_t)8 * n);
+        double my_err = 0.0;
 #pragma omp for schedule(dynamic, 1)
         for (int cell = 0; cell < nc; ++cell) {
             if (status != OR_OK) continue;
-            const double *W = s->W + 4 * cell;
+            const double *W = Wsrc + 4 * cell;
             double avg[4] = {0, 0, 0, 0};
             for (int cc = 0; cc < 3; ++cc)
                 for (int a = 0; a < 3; ++a)
                     for (int m = 0; m < 4; ++m) avg[m] += GW[a] * GW[cc] * nodes[((size_t)cell * 4 + m) * 9 + a * 3 + cc];
-            double *mh = s->mh + (size_t)cell * n, *mb = s->mb + (size_t)cell * n, *kxh = s->kxh + (size_t)cell * n,
-                   *kxb = s->kxb + (size_t)cell * n, *kyh = s->kyh + (size_t)cell * n, *kyb = s->kyb + (size_t)cell * n;
+            double *mh = omh + (size_t)cell * n, *mb = omb + (size_t)cell * n, *kxh = okxh + (size_t)cell * n,
+                   *kxb = okxb + (size_t)cell * n, *kyh = okyh + (size_t)cell * n, *kyb = okyb + (size_t)cell * n;
             for (int k = 0; k < n; ++k) mh[k] = mb[k] = kxh[k] = kxb[k] = kyh[k] = kyb[k] = 0.0;
             for (int cc = 0; cc < 3; ++cc)
                 for (int a = 0; a < 3; ++a) {
@@ -1343,18 +1351,31 @@ static int build_targets(or_solver *s) {
                         status = OR_GATE;
                         break;
                     }
+                    if (res > my_err) my_err = res;
                     for (int k = 0; k < n; ++k) {
                         mh[k] += wt2 * t[k]; mb[k] += wt2 * t[n + k];
                         kxh[k] += wt2 * t[2 * n + k]; kxb[k] += wt2 * t[3 * n + k];
                         kyh[k] += wt2 * t[4 * n + k]; kyb[k] += wt2 * t[5 * n + k];
                     }
                 }
+            if (okappa && status == OR_OK) {
+                if (!s2_heat(W, P->ck, n, kxh, kxb, s->u, s->v, s->w, okappa + 4 * (size_t)cell) ||
+                    !s2_heat(W, P->ck, n, kyh, kyb, s->u, s->v, s->w, okappa + 4 * (size_t)cell + 2))
+                    status = OR_NONFINITE;
+            }
         }
+#pragma omp critical
+        if (my_err > terr) terr = my_err;
         free(t);
         closure_ws_free(&c);
     }
     free(nodes);
+    if (target_err) *target_err = terr;
     return status;
+}
+
+static int build_targets(or_solver *s) {
+    return build_targets_of(s, s->W, s->mh, s->mb, s->kxh, s->kxb, s->kyh, s->kyb, NULL, NULL);
 }
 
 /* ComputeResidual (AssembleMacroResidual, AssembleMicroFluxResidual,
@@ -2059,4 +2080,180 @@ int or_make_mke(const double w[4], int ck, double pr, int nvel, const double *u,
     free(tmp);
     closure_ws_free(&c);
     return ok ? OR_OK : OR_GATE;
+}
+
+/* ------------------------------------------------------------------------ */
+/* S2CN2 explicit step                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* One S2CN2 step: advance_step with the physical gas (src/mod_S2D1.f90:1302-1435)
+   fed by the native flux residual (src/mod_S2NativeAdapter.f90:877-905), then
+   the global conservation gate of S2NativeApplyUpdate with the wall ledger
+   (:925-948).  The old targets are those of the current state (the reference
+   caches them from the previous step's build_targets of the same state).
+   diag16: [0] face error (0), [1] target error, [2] CN residual, [3] moment
+   error, [4] limited (0), [5] signed (0), [6] max 2 beta, [7] CFL number,
+   [8] min hn, [9] max hn, [10] min bn, [11] max bn, [12] drift, [13] dt,
+   [14] rejected by the collision/target gate, [15] rejected by the drift gate. */
+int or_cn_step(or_solver *s, int *accepted, double *diag) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell, ck = P->ck;
+    *accepted = 0;
+    for (int q = 0; q < 16; ++q) diag[q] = 0.0;
+    if (!s->cn_ledger) {
+        for (int m = 0; m < 4; ++m) { s->cn_total0[m] = 0.0; s->cn_balance[m] = 0.0; }
+        for (int c = 0; c < nc; ++c)
+            for (int m = 0; m < 4; ++m) s->cn_total0[m] += s->W[4 * c + m];
+        s->cn_ledger = 1;
+    }
+    s->dt = time_step(s);
+    const double dt = s->dt;
+    int st = build_targets(s);   /* old targets: those of the current state */
+    if (st != OR_OK) return st;
+    st = compute_flux(s);
+    if (st != OR_OK) return st;
+    /* streamed state from the flux residual */
+    double *wn = dalloc((size_t)nc * 4), *hs = dalloc((size_t)nc * n), *bs = dalloc((size_t)nc * n);
+    const double area = P->dx * P->dy, ainv = 1.0 / area;
+    for (int c = 0; c < nc; ++c) {
+        int j = c / nx, i = c % nx, cw = j * nx + ip(i - 1, nx);
+        size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
+        for (int m = 0; m < 4; ++m)
+            wn[4 * c + m] = s->W[4 * c + m] + (s->fxw[4 * cw + m] * P->dy - s->fxw[4 * c + m] * P->dy +
+                                               s->fyw[4 * fs + m] * P->dx - s->fyw[4 * fn + m] * P->dx) / area;
+        for (int k = 0; k < n; ++k) {
+            size_t o = (size_t)c * n + k;
+            hs[o] = s->h[o] + (s->fxh[(size_t)cw * n + k] * P->dy - s->fxh[o] * P->dy + s->fyh[fs * n + k] * P->dx -
+                               s->fyh[fn * n + k] * P->dx) * ainv;
+            bs[o] = s->b[o] + (s->fxb[(size_t)cw * n + k] * P->dy - s->fxb[o] * P->dy + s->fyb[fs * n + k] * P->dx -
+                               s->fyb[fn * n + k] * P->dx) * ainv;
+        }
+    }
+    /* new targets at wn */
+    double *mhn = dalloc((size_t)nc * n), *mbn = dalloc((size_t)nc * n), *kxhn = dalloc((size_t)nc * n),
+           *kxbn = dalloc((size_t)nc * n), *kyhn = dalloc((size_t)nc * n), *kybn = dalloc((size_t)nc * n),
+           *kap = dalloc((size_t)nc * 4), *hn = dalloc((size_t)nc * n), *bn = dalloc((size_t)nc * n);
+    double target_err = 0.0;
+    st = build_targets_of(s, wn, mhn, mbn, kxhn, kxbn, kyhn, kybn, kap, &target_err);
+    double cnres = 0.0, mmerr = 0.0, two_beta = 0.0;
+    double hmin = 1e300, hmax = -1e300, bmin = 1e300, bmax = -1e300;
+    if (st == OR_OK) {
+#pragma omp parallel
+        {
+            double *rh = dalloc(n), *rb = dalloc(n), *eqh = dalloc(n), *eqb = dalloc(n);
+            double my_cn = 0.0, my_mm = 0.0, my_tb = 0.0, mhmin = 1e300, mhmax = -1e300, mbmin = 1e300, mbmax = -1e300;
+            int my_st = OR_OK;
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                if (my_st != OR_OK) continue;
+                const double *w = s->W + 4 * c, *wnc = wn + 4 * c;
+                size_t o = (size_t)c * n;
+                const double *h = s->h + o, *b = s->b + o;
+                double beta_old = dt / (2.0 * s2_tau(P, w)), beta = dt / (2.0 * s2_tau(P, wnc));
+                if (2.0 * beta_old > my_tb) my_tb = 2.0 * beta_old;
+                if (2.0 * beta > my_tb) my_tb = 2.0 * beta;
+                double rold[2], rhs[2], qtmp[2], rnew[2], rhat[2];
+                if (!s2_heat_diff(w, ck, n, h, s->mh + o, b, s->mb + o, s->u, s->v, s->w, rold)) {
+                    my_st = OR_NONFINITE;
+                    continue;
+                }
+                for (int k = 0; k < n; ++k) {
+                    rh[k] = hs[o + k] + beta_old * (s->mh[o + k] + s->kxh[o + k] * rold[0] + s->kyh[o + k] * rold[1] - h[k]);
+                    rb[k] = bs[o + k] + beta_old * (s->mb[o + k] + s->kxb[o + k] * rold[0] + s->kyb[o + k] * rold[1] - b[k]);
+                }
+                if (!s2_heat(wnc, ck, n, rh, rb, s->u, s->v, s->w, rhs) ||
+                    !s2_heat(wnc, ck, n, mhn + o, mbn + o, s->u, s->v, s->w, qtmp)) {
+                    my_st = OR_NONFINITE;
+                    continue;
+                }
+                rhs[0] -= qtmp[0];
+                rhs[1] -= qtmp[1];
+                /* a22 = (1+beta) I - beta kappa, kappa(:,1) = heat(kx), kappa(:,2) = heat(ky) */
+                const double *K = kap + 4 * c;
+                double a11 = 1.0 + beta - beta * K[0], a21 = -beta * K[1];
+                double a12 = -beta * K[2], a22 = 1.0 + beta - beta * K[3];
+                double det = a11 * a22 - a12 * a21;
+                if (!isfinite(det) || det <= 0.0) { my_st = OR_GATE; continue; }
+                rnew[0] = (rhs[0] * a22 - a12 * rhs[1]) / det;
+                rnew[1] = (a11 * rhs[1] - rhs[0] * a21) / det;
+                double scale = 1.0, e1 = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    eqh[k] = mhn[o + k] + kxhn[o + k] * rnew[0] + kyhn[o + k] * rnew[1];
+                    eqb[k] = mbn[o + k] + kxbn[o + k] * rnew[0] + kybn[o + k] * rnew[1];
+                    hn[o + k] = (rh[k] + beta * eqh[k]) / (1.0 + beta);
+                    bn[o + k] = (rb[k] + beta * eqb[k]) / (1.0 + beta);
+                    scale = fmax(scale, fmax(fmax(fabs(rh[k]), fabs(rb[k])), fmax(fabs(hn[o + k]), fabs(bn[o + k]))));
+                    e1 = fmax(e1, fmax(fabs((1 + beta) * hn[o + k] - rh[k] - beta * eqh[k]),
+                                       fabs((1 + beta) * bn[o + k] - rb[k] - beta * eqb[k])));
+                    if (hn[o + k] < mhmin) mhmin = hn[o + k];
+                    if (hn[o + k] > mhmax) mhmax = hn[o + k];
+                    if (bn[o + k] < mbmin) mbmin = bn[o + k];
+                    if (bn[o + k] > mbmax) mbmax = bn[o + k];
+                }
+                if (e1 / scale > my_cn) my_cn = e1 / scale;
+                if (!s2_heat_diff(wnc, ck, n, hn + o, mhn + o, bn + o, mbn + o, s->u, s->v, s->w, rhat)) {
+                    my_st = OR_NONFINITE;
+                    continue;
+                }
+                double rs = fmax(1.0, fmax(fmax(fabs(rhat[0]), fabs(rhat[1])), fmax(fabs(rnew[0]), fabs(rnew[1]))));
+                double e2 = fmax(fabs(rhat[0] - rnew[0]), fabs(rhat[1] - rnew[1])) / rs;
+                if (e2 > my_cn) my_cn = e2;
+                double mom[4];
+                if (!moment_2d(n, hn + o, bn + o, s->u, s->v, s->w, mom)) { my_st = OR_NONFINITE; continue; }
+                double wm = 1.0, em = 0.0;
+                for (int m = 0; m < 4; ++m) { wm = fmax(wm, fabs(wnc[m])); em = fmax(em, fabs(mom[m] - wnc[m])); }
+                if (em / wm > my_mm) my_mm = em / wm;
+            }
+#pragma omp critical
+            {
+                if (my_st != OR_OK && st == OR_OK) st = my_st;
+                cnres = fmax(cnres, my_cn); mmerr = fmax(mmerr, my_mm); two_beta = fmax(two_beta, my_tb);
+                hmin = fmin(hmin, mhmin); hmax = fmax(hmax, mhmax); bmin = fmin(bmin, mbmin); bmax = fmax(bmax, mbmax);
+            }
+            free(rh); free(rb); free(eqh); free(eqb);
+        }
+    }
+    double umax = 0.0, vmax = 0.0;
+    for (int k = 0; k < n; ++k) { umax = fmax(umax, fabs(s->u[k])); vmax = fmax(vmax, fabs(s->v[k])); }
+    diag[1] = target_err; diag[2] = cnres; diag[3] = mmerr; diag[6] = two_beta;
+    diag[7] = dt * (umax / P->dx + vmax / P->dy);
+    diag[8] = hmin; diag[9] = hmax; diag[10] = bmin; diag[11] = bmax; diag[13] = dt;
+    int finite = 1;
+    for (size_t q = 0; q < (size_t)nc * 4 && finite; ++q) finite = isfinite(wn[q]);
+    for (size_t q = 0; q < (size_t)nc * n && finite; ++q) finite = isfinite(hn[q]) && isfinite(bn[q]);
+    int ok = st == OR_OK && finite && cnres <= GATE && mmerr <= GATE;
+    if (!ok) {
+        diag[14] = 1.0;
+    } else {
+        /* conservation with the wall ledger (S2W1_wall_mode = 1) */
+        double total[4] = {0, 0, 0, 0}, bottom[4] = {0, 0, 0, 0}, top[4] = {0, 0, 0, 0}, next[4], drift = 0.0;
+        for (int c = 0; c < nc; ++c)
+            for (int m = 0; m < 4; ++m) total[m] += wn[4 * c + m];
+        for (int i = 0; i < nx; ++i)
+            for (int m = 0; m < 4; ++m) {
+                bottom[m] += s->fyw[4 * (size_t)i + m] / P->dy;
+                top[m] += s->fyw[4 * ((size_t)ny * nx + i) + m] / P->dy;
+            }
+        for (int m = 0; m < 4; ++m) {
+            next[m] = s->cn_balance[m] + top[m] - bottom[m];
+            drift = fmax(drift, fabs(total[m] + next[m] - s->cn_total0[m]) / fmax(1.0, fabs(s->cn_total0[m])));
+        }
+        diag[12] = drift;
+        if (drift > GATE) {
+            diag[15] = 1.0;
+        } else {
+            memcpy(s->W, wn, sizeof(double) * nc * 4);
+            memcpy(s->h, hn, sizeof(double) * nc * n);
+            memcpy(s->b, bn, sizeof(double) * nc * n);
+            for (int m = 0; m < 4; ++m) s->cn_balance[m] = next[m];
+            *accepted = 1;
+        }
+    }
+    free(wn); free(hs); free(bs); free(mhn); free(mbn); free(kxhn); free(kxbn); free(kyhn); free(kybn);
+    free(kap); free(hn); free(bn);
+    return st == OR_OK ? (finite ? OR_OK : OR_NONFINITE) : st;
+}
+
+void or_cn_ledger(const or_solver *s, double *out8) {
+    for (int m = 0; m < 4; ++m) { out8[m] = s->cn_total0[m]; out8[4 + m] = s->cn_balance[m]; }
 }

@@ -74,6 +74,12 @@ int or_make_mke(const double w[4], int ck, double pr, int nvel, const double *u,
                 const double *wt, double *mh, double *mb, double *kxh, double *kxb, double *kyh,
                 double *kyb, double kappa[4], double *residual);
 
+/* S2CN2 explicit step (advance_step + the conservation gate of
+   S2NativeApplyUpdate); diag16 layout in ugks_oracle.c */
+int or_cn_step(or_solver *s, int *accepted, double *diag16);
+/* S2CN2 conservation ledger: total0[4] then wall balance[4] */
+void or_cn_ledger(const or_solver *s, double *out8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ EXPORTS = [
     "ugks_create", "ugks_destroy", "ugks_last_error", "ugks_set_state", "ugks_get_state", "ugks_cold_start",
     "ugks_evolution", "ugks_norms", "ugks_implicit_step", "ugks_run", "ugks_observables",
     "ugks_set_state_device", "ugks_get_state_device", "ugks_sync", "ugks_debug_fetch",
-    "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning",
+    "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning", "ugks_cn_step", "ugks_cn_ledger",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -71,6 +71,8 @@ def load_library(path: str = LIB_PATH):
     L.ugks_norms.argtypes = [C.c_void_p, _D, _D]
     L.ugks_implicit_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
     L.ugks_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.ugks_cn_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+    L.ugks_cn_ledger.argtypes = [C.c_void_p, _D]
     L.ugks_observables.argtypes = [C.c_void_p, _D]
     L.ugks_sync.argtypes = [C.c_void_p]
     L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
@@ -182,6 +184,19 @@ class UGKS:
         d = np.zeros(16)
         _check(load_library().ugks_implicit_step(self.handle, C.byref(acc), _ptr(d)))
         return bool(acc.value), d
+
+    def cn_step(self):
+        """One S2CN2 explicit step; returns (accepted, diag16).  A gate
+        rejection leaves the state unchanged and returns accepted False."""
+        acc = C.c_int(0)
+        d = np.zeros(16)
+        _check(load_library().ugks_cn_step(self.handle, C.byref(acc), _ptr(d)))
+        return bool(acc.value), d
+
+    def cn_ledger(self):
+        o = np.zeros(8)
+        _check(load_library().ugks_cn_ledger(self.handle, _ptr(o)))
+        return o[:4], o[4:]
 
     def run(self, n: int) -> int:
         done = C.c_int(0)

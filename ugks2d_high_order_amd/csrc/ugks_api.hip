@@ -22,6 +22,7 @@ extern "C" hipError_t ugks_launch_operator(const Dev* D, double umax, double vma
                                            Prof* P);
 extern "C" hipError_t ugks_launch_implicit(const Dev* D, int rescale, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_wall_reflection(const Dev* D, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_cn(const Dev* D, int build_old, double umax, double vmax, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
 extern "C" hipError_t ugks_launch_observables(const Dev* D, double* out, hipStream_t s);
 
@@ -51,6 +52,8 @@ struct Instance {
     std::vector<void*> allocs;
     double* obs = nullptr;
     Prof prof;
+    bool cn_alloc = false;        // S2CN2 target buffers allocated
+    bool cn_targets_fresh = false; // tg holds the volume targets of the current state
 };
 
 // Polynomial coefficients (degrees 0..n-1) whose averages over the unit cells
@@ -239,6 +242,15 @@ extern "C" void ugks_destroy(void* handle) {
     delete I;
 }
 
+// a new state starts a new S2CN2 run: its targets and conservation ledger
+static int reset_cn(Instance* I) {
+    static const double zero = 0.0;
+    I->cn_targets_fresh = false;
+    HIPCHK(hipMemcpyAsync(I->D.scal + SC_CN_INIT, &zero, sizeof(double), hipMemcpyHostToDevice, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
 static int set_state_impl(Instance* I, const double* W, const double* h, const double* b, hipMemcpyKind kind) {
     const size_t nc = I->D.ncell, cv = nc * I->D.nvel;
     HIPCHK(hipMemcpyAsync(I->D.W, W, sizeof(double) * 4 * nc, kind, I->stream));
@@ -246,7 +258,7 @@ static int set_state_impl(Instance* I, const double* W, const double* h, const d
     HIPCHK(hipMemcpyAsync(I->D.b, b, sizeof(double) * cv, kind, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));
     I->operator_fresh = false;
-    return E_OK;
+    return reset_cn(I);
 }
 
 static int get_state_impl(Instance* I, double* W, double* h, double* b, hipMemcpyKind kind) {
@@ -284,7 +296,8 @@ extern "C" int ugks_cold_start(void* handle, double rho, double ux, double t_nat
     if (!(rho > 0.0) || !(t_native > 0.0)) return fail(E_ARG, "cold start needs rho > 0 and T > 0");
     HIPCHK(ugks_launch_cold_start(&I->D, rho, ux, t_native, I->stream));
     I->operator_fresh = false;
-    return check_status(I, nullptr);
+    int rc = check_status(I, nullptr);
+    return rc ? rc : reset_cn(I);
 }
 
 extern "C" int ugks_sync(void* handle) {
@@ -358,7 +371,60 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
     std::swap(I->D.h, I->D.hn);
     std::swap(I->D.b, I->D.bn);
     I->operator_fresh = false;
+    I->cn_targets_fresh = false;
     *accepted = 1;
+    return E_OK;
+}
+
+// S2CN2 explicit step (no counterpart as one call in the reference: it is
+// S2NativeEvolution's fluxes followed by S2NativeApplyUpdate, i.e. advance_step
+// with the conservation gate, src/mod_S2NativeAdapter.f90:469-599,867-1018).
+extern "C" int ugks_cn_step(void* handle, int* accepted, double* diag16) {
+    Instance* I = (Instance*)handle;
+    if (!I || !accepted) return fail(E_ARG, "null argument");
+    *accepted = 0;
+    Dev& D = I->D;
+    if (!I->cn_alloc) {
+        const size_t cv = (size_t)D.ncell * D.nvel;
+        int rc;
+        for (int q = 0; q < 6; ++q)
+            if ((rc = dmalloc(I, &D.tg[q], cv)) || (rc = dmalloc(I, &D.tgn[q], cv))) return rc;
+        if ((rc = dmalloc(I, &D.cnpart, (size_t)D.ncell * 8))) return rc;
+        I->cn_alloc = true;
+        I->cn_targets_fresh = false;
+    }
+    HIPCHK(ugks_launch_cn(&D, I->cn_targets_fresh ? 0 : 1, I->umax, I->vmax, I->stream, &I->prof));
+    double sc[SC_COUNT];
+    unsigned st = 0;
+    HIPCHK(hipMemcpyAsync(sc, D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
+    int rc = check_status(I, &st);
+    if (diag16)
+        for (int a = 0; a < 16; ++a) diag16[a] = sc[SC_CN_DIAG + a];
+    if (rc) {
+        // the state is untouched; the old targets were built only if valid
+        if (!(st & ST_AT_CELL)) I->cn_targets_fresh = true;
+        return rc;
+    }
+    I->cn_targets_fresh = true;   // tg now describes the current state
+    I->operator_fresh = false;
+    if (sc[SC_CN_ACCEPT] != 1.0) return E_OK;
+    std::swap(D.W, D.Wn);
+    std::swap(D.h, D.hn);
+    std::swap(D.b, D.bn);
+    for (int q = 0; q < 6; ++q) std::swap(D.tg[q], D.tgn[q]);
+    *accepted = 1;
+    return E_OK;
+}
+
+// S2CN2 conservation ledger: totals at the start of the run (4) and the
+// cumulative wall balance (4).
+extern "C" int ugks_cn_ledger(void* handle, double* out8) {
+    Instance* I = (Instance*)handle;
+    if (!I || !out8) return fail(E_ARG, "null argument");
+    double sc[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(sc, I->D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    for (int m = 0; m < 4; ++m) { out8[m] = sc[SC_CN_TOTAL0 + m]; out8[4 + m] = sc[SC_CN_BAL + m]; }
     return E_OK;
 }
 

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vma
 // ===========================================================================
 // One workgroup row per cell row (blockIdx.y = j), so the wall-stencil choice
 // is uniform over the workgroup and the near-wall branch never diverges.
-__global__ void __launch_bounds__(256) k_wrecon(Dev D) {
+__global__ void __launch_bounds__(256) k_wrecon(Dev D, const double* Wsrc, double* ws_out, double* wv_out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 4 * D.nx) return;
     const int j = blockIdx.y, i = t >> 2, m = t & 3, c = j * D.nx + i;
@@ -55,11 +55,11 @@ __global__ void __launch_bounds__(256) k_wrecon(Dev D) {
         for (int s = 0; s < 5; ++s) {
             int col = i + s - 2;
             col = col < 0 ? col + D.nx : (col >= D.nx ? col - D.nx : col);
-            blk[r][s] = D.W[4 * ((rb + r) * D.nx + col) + m];
+            blk[r][s] = Wsrc[4 * ((rb + r) * D.nx + col) + m];
         }
     FirstLayer F;
     first_layer(KC, blk, near, sh, F);
-    double* o = D.ws + (size_t)(c * 4 + m) * 24;
+    double* o = ws_out + (size_t)(c * 4 + m) * 24;
     for (int s = 0; s < 4; ++s) {
         double d[3][3];
         side_data(KC, F, near, sh, s, d);
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_wrecon(Dev D) {
     }
     double vol[3][3];
     vol_data(KC, F, near, sh, vol);
-    double* ov = D.wv + (size_t)(c * 4 + m) * 9;
+    double* ov = wv_out + (size_t)(c * 4 + m) * 9;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) ov[a * 3 + b] = vol[a][b];
 }
@@ -737,38 +737,33 @@ __global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
 // Per volume node the closure state and the projected Shakhov directions are
 // kept in LDS (uniform values); the averaged Maxwellian (mh, mb) is kept per
 // velocity in LDS slots.  The averaged Shakhov kernels are rebuilt per
-// velocity in the final pass from the node states (q recomputed from theta).
-enum { CN_TH = 0, CN_UX = 4, CN_VY = 5, CN_LAM = 6, CN_COEFF = 7, CN_TDX = 8, CN_TDY = 12, CN_WT = 16, CN_PRE = 17,
-       CN_IT = 18, CN_I0 = 19, CN_LEN = 20 };
+// velocity from the node states (q recomputed from theta).
+enum { VN_TH = 0, VN_UX = 4, VN_VY = 5, VN_LAM = 6, VN_COEFF = 7, VN_TDX = 8, VN_TDY = 12, VN_WT = 16, VN_PRE = 17,
+       VN_IT = 18, VN_I0 = 19, VN_LEN = 20 };
 
+// Volume targets of one cell (build_targets, src/mod_S2D1.f90:1246-1300): the
+// 3x3 volume nodes of the WENO tensor of W (wvc, [m][9]), mean-corrected,
+// each closed by the Newton solve.  Leaves the node states in node[] (written
+// by thread 0) and the Gauss-averaged Maxwellian in the LDS slots amh, amb.
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cell(Dev D) {
-    __shared__ double lds[(NT / 16) * 20];
-    __shared__ double node[9 * CN_LEN];
-    UGKS_DYN_SHARED(double, dyn);
-    const int nx = D.nx, nvel = D.nvel, ck = D.gas.ck;
-    double* amh = dyn;
-    double* amb = dyn + nvel;
-    const int c = blockIdx.x, i = c % nx, j = c / nx;
-    const double* Wc = D.W + 4 * c;
-    double W[4] = {Wc[0], Wc[1], Wc[2], Wc[3]};
-    // volume nodes of W and their Gauss mean
+__device__ inline unsigned volume_targets(const Dev& D, const double W[4], const double* wvc, double* node,
+                                          double* amh, double* amb, double* lds) {
+    const int nvel = D.nvel, ck = D.gas.ck;
     double nodes[4][9], avg[4] = {0, 0, 0, 0};
     for (int m = 0; m < 4; ++m)
-        for (int q = 0; q < 9; ++q) nodes[m][q] = D.wv[(size_t)(c * 4 + m) * 9 + q];
+        for (int q = 0; q < 9; ++q) nodes[m][q] = wvc[m * 9 + q];
     for (int cc = 0; cc < 3; ++cc)
         for (int a = 0; a < 3; ++a)
             for (int m = 0; m < 4; ++m) avg[m] += KC.gw[a] * KC.gw[cc] * nodes[m][a * 3 + cc];
-    unsigned st = 0;
-    for (int n = 0; n < 9 && st == 0; ++n) {
+    for (int n = 0; n < 9; ++n) {
         const int cc = n / 3, a = n % 3;
         double wq[4];
         for (int m = 0; m < 4; ++m) wq[m] = nodes[m][a * 3 + cc] + W[m] - avg[m];
         const double wt2 = KC.gw[a] * KC.gw[cc];
         double th[4], J[16];
-        if (!closure_solve<NT, false>(ck, D, wq, th, J, nullptr, lds)) { st = ST_NEWTON; break; }
+        if (!closure_solve<NT, false>(ck, D, wq, th, J, nullptr, lds)) return ST_NEWTON;
         double rho, ux, vy, thg;
-        if (!s2_prim(ck, wq, &rho, &ux, &vy, &thg)) { st = ST_NONFINITE; break; }
+        if (!s2_prim(ck, wq, &rho, &ux, &vy, &thg)) return ST_NONFINITE;
         const double lam = 0.5 / thg, coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho, T = th[3];
         const double pre = th[0] / (UGKS_PI * T), iT = 1.0 / T;
         double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -784,17 +779,61 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
             amh[k] = (n == 0 ? 0.0 : amh[k]) + wt2 * mh;
             amb[k] = (n == 0 ? 0.0 : amb[k]) + wt2 * mb;
         }
-        if (!__syncthreads_and(pos)) { st = ST_NEWTON; break; }
+        if (!__syncthreads_and(pos)) return ST_NEWTON;
         block_sum<8>(red, lds);
         double tdx[4], tdy[4];
-        if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) { st = ST_SOLVE; break; }
+        if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) return ST_SOLVE;
         if (threadIdx.x == 0) {
-            double* nd = node + n * CN_LEN;
-            for (int q = 0; q < 4; ++q) { nd[CN_TH + q] = th[q]; nd[CN_TDX + q] = tdx[q]; nd[CN_TDY + q] = tdy[q]; }
-            nd[CN_UX] = ux; nd[CN_VY] = vy; nd[CN_LAM] = lam; nd[CN_COEFF] = coeff; nd[CN_WT] = wt2; nd[CN_PRE] = pre;
-            nd[CN_IT] = iT; nd[CN_I0] = 1.0 / th[0];
+            double* nd = node + n * VN_LEN;
+            for (int q = 0; q < 4; ++q) { nd[VN_TH + q] = th[q]; nd[VN_TDX + q] = tdx[q]; nd[VN_TDY + q] = tdy[q]; }
+            nd[VN_UX] = ux; nd[VN_VY] = vy; nd[VN_LAM] = lam; nd[VN_COEFF] = coeff; nd[VN_WT] = wt2; nd[VN_PRE] = pre;
+            nd[VN_IT] = iT; nd[VN_I0] = 1.0 / th[0];
         }
     }
+    __syncthreads();   // node[] written by thread 0
+    return 0;
+}
+
+// Gauss-averaged projected Shakhov kernels of velocity (u, v) from the node states
+__device__ inline void target_kernels(const double* node, int ck, double u, double v, double* akxh, double* akxb,
+                                      double* akyh, double* akyb) {
+    double sxh = 0.0, sxb = 0.0, syh = 0.0, syb = 0.0;
+#pragma unroll 1
+    for (int n = 0; n < 9; ++n) {
+        const double* nd = node + n * VN_LEN;
+        const double th[4] = {nd[VN_TH], nd[VN_TH + 1], nd[VN_TH + 2], nd[VN_TH + 3]};
+        const double nux = nd[VN_UX], nvy = nd[VN_VY], lam = nd[VN_LAM], coeff = nd[VN_COEFF], wt2 = nd[VN_WT];
+        const double mh = q_of(th, nd[VN_PRE], nd[VN_IT], u, v), mb = 0.5 * (double)ck * th[3] * mh;
+        double qth[4], qtb[4];
+        qtheta(ck, th, nd[VN_I0], nd[VN_IT], u, v, mh, qth, qtb);
+        double cx = u - nux, cy = v - nvy, c2 = cx * cx + cy * cy;
+        double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+        double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
+        double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
+        double dxh = 0, dxb = 0, dyh = 0, dyb = 0;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            dxh += qth[qq] * nd[VN_TDX + qq]; dxb += qtb[qq] * nd[VN_TDX + qq];
+            dyh += qth[qq] * nd[VN_TDY + qq]; dyb += qtb[qq] * nd[VN_TDY + qq];
+        }
+        sxh += wt2 * (kxh - dxh); sxb += wt2 * (kxb - dxb);
+        syh += wt2 * (kyh - dyh); syb += wt2 * (kyb - dyb);
+    }
+    *akxh = sxh; *akxb = sxb; *akyh = syh; *akyb = syb;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_cell(Dev D) {
+    __shared__ double lds[(NT / 16) * 20];
+    __shared__ double node[9 * VN_LEN];
+    UGKS_DYN_SHARED(double, dyn);
+    const int nx = D.nx, nvel = D.nvel, ck = D.gas.ck;
+    double* amh = dyn;
+    double* amb = dyn + nvel;
+    const int c = blockIdx.x, i = c % nx, j = c / nx;
+    const double* Wc = D.W + 4 * c;
+    double W[4] = {Wc[0], Wc[1], Wc[2], Wc[3]};
+    unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds);
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
@@ -814,7 +853,6 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
         q[0] += 0.5 * w * cx * (c2 * dh + db);
         q[1] += 0.5 * w * cy * (c2 * dh + db);
     }
-    __syncthreads();   // node[] written by thread 0
     block_sum<2>(q, lds);
     const double tau = tau_of_w(D.gas, W), itau = 1.0 / tau;
     const double dt = D.scal[SC_DT], inv_dt = 1.0 / dt;
@@ -825,28 +863,8 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
     int fin = 1;
     UGKS_VLOOP(k, NT, nvel) {
         const double u = D.u[k], v = D.v[k], w = D.w[k];
-        double akxh = 0.0, akxb = 0.0, akyh = 0.0, akyb = 0.0;
-#pragma unroll 1
-        for (int n = 0; n < 9; ++n) {
-            const double* nd = node + n * CN_LEN;
-            const double th[4] = {nd[CN_TH], nd[CN_TH + 1], nd[CN_TH + 2], nd[CN_TH + 3]};
-            const double nux = nd[CN_UX], nvy = nd[CN_VY], lam = nd[CN_LAM], coeff = nd[CN_COEFF], wt2 = nd[CN_WT];
-            const double mh = q_of(th, nd[CN_PRE], nd[CN_IT], u, v), mb = 0.5 * (double)ck * th[3] * mh;
-            double qth[4], qtb[4];
-            qtheta(ck, th, nd[CN_I0], nd[CN_IT], u, v, mh, qth, qtb);
-            double cx = u - nux, cy = v - nvy, c2 = cx * cx + cy * cy;
-            double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
-            double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
-            double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
-            double dxh = 0, dxb = 0, dyh = 0, dyb = 0;
-#pragma unroll
-            for (int qq = 0; qq < 4; ++qq) {
-                dxh += qth[qq] * nd[CN_TDX + qq]; dxb += qtb[qq] * nd[CN_TDX + qq];
-                dyh += qth[qq] * nd[CN_TDY + qq]; dyb += qtb[qq] * nd[CN_TDY + qq];
-            }
-            akxh += wt2 * (kxh - dxh); akxb += wt2 * (kxb - dxb);
-            akyh += wt2 * (kyh - dyh); akyb += wt2 * (kyb - dyb);
-        }
+        double akxh, akxb, akyh, akyb;
+        target_kernels(node, ck, u, v, &akxh, &akxb, &akyh, &akyb);
         const double hv = D.h[cb0 + k], bv = D.b[cb0 + k];
         double eh = amh[k] + akxh * q[0] + akyh * q[1];
         double eb = amb[k] + akxb * q[0] + akyb * q[1];
@@ -889,6 +907,211 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
         cp[13] = rw[0] * area;
         if (st) atomicOr(D.status, st | ST_AT_CELL);
     }
+}
+
+// ===========================================================================
+// S2CN2 explicit step: advance_step with the physical gas
+// (src/mod_S2D1.f90:1302-1435) fed by the native flux residual
+// (src/mod_S2NativeAdapter.f90:877-905), and its conservation gate (:925-948)
+// ===========================================================================
+// streamed state: Wn = W + flux residual, hs/bs (into rh/rb) = h/b + flux residual
+__global__ void __launch_bounds__(256) k_cn_stream(Dev D) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y, nx = D.nx, nvel = D.nvel;
+    const int j = c / nx, i = c % nx, cw = j * nx + (i == 0 ? nx - 1 : i - 1);
+    const size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
+    const double area = D.dx * D.dy, ainv = 1.0 / area;
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        const int m = threadIdx.x;
+        D.Wn[4 * (size_t)c + m] = D.W[4 * (size_t)c + m] +
+                                  (D.xfw[4 * (size_t)cw + m] * D.dy - D.xfw[4 * (size_t)c + m] * D.dy +
+                                   D.yfw[4 * fs + m] * D.dx - D.yfw[4 * fn + m] * D.dx) / area;
+    }
+    if (k >= nvel) return;
+    const size_t o = (size_t)c * nvel + k;
+    D.rh[o] = D.h[o] + (D.xfl_h[(size_t)cw * nvel + k] * D.dy - D.xfl_h[o] * D.dy + D.yfl_h[fs * nvel + k] * D.dx -
+                        D.yfl_h[fn * nvel + k] * D.dx) * ainv;
+    D.rb[o] = D.b[o] + (D.xfl_b[(size_t)cw * nvel + k] * D.dy - D.xfl_b[o] * D.dy + D.yfl_b[fs * nvel + k] * D.dx -
+                        D.yfl_b[fn * nvel + k] * D.dx) * ainv;
+}
+
+// volume targets of the state W into tg (first step of an S2CN2 run)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_cn_targets(Dev D) {
+    __shared__ double lds[(NT / 16) * 20];
+    __shared__ double node[9 * VN_LEN];
+    UGKS_DYN_SHARED(double, dyn);
+    const int nvel = D.nvel, ck = D.gas.ck, c = blockIdx.x;
+    double* amh = dyn;
+    double* amb = dyn + nvel;
+    double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
+    unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds);
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
+        return;
+    }
+    const size_t cb0 = (size_t)c * nvel;
+    UGKS_VLOOP(k, NT, nvel) {
+        double akxh, akxb, akyh, akyb;
+        target_kernels(node, ck, D.u[k], D.v[k], &akxh, &akxb, &akyh, &akyb);
+        D.tg[0][cb0 + k] = amh[k]; D.tg[1][cb0 + k] = amb[k];
+        D.tg[2][cb0 + k] = akxh; D.tg[3][cb0 + k] = akxb;
+        D.tg[4][cb0 + k] = akyh; D.tg[5][cb0 + k] = akyb;
+    }
+}
+
+// heat flux contribution of one velocity relative to the bulk velocity (ux, vy)
+__device__ inline void heat_add(double q[2], double w, double cx, double cy, double h, double b) {
+    const double c2 = cx * cx + cy * cy;
+    q[0] += 0.5 * w * cx * (c2 * h + b);
+    q[1] += 0.5 * w * cy * (c2 * h + b);
+}
+
+// Crank-Nicolson Shakhov collision of one cell: new targets at Wn (written to
+// tgn), the 2x2 heat-flux solve and hn, bn; per-cell gate quantities into cnpart.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_cn_cell(Dev D) {
+    __shared__ double lds[(NT / 16) * 20];
+    __shared__ double node[9 * VN_LEN];
+    UGKS_DYN_SHARED(double, dyn);
+    const int nvel = D.nvel, ck = D.gas.ck, c = blockIdx.x;
+    double* amh = dyn;
+    double* amb = dyn + nvel;
+    const double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
+    const double Wn[4] = {D.Wn[4 * c], D.Wn[4 * c + 1], D.Wn[4 * c + 2], D.Wn[4 * c + 3]};
+    unsigned st = volume_targets<NT>(D, Wn, D.wv + (size_t)c * 36, node, amh, amb, lds);
+    if (st) {
+        if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
+        return;
+    }
+    double rho, ux, vy, thg, rho_o, ux_o, vy_o, thg_o;
+    if (!s2_prim(ck, Wn, &rho, &ux, &vy, &thg) || !s2_prim(ck, W, &rho_o, &ux_o, &vy_o, &thg_o)) {
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_NONFINITE | ST_AT_CELL));
+        return;
+    }
+    const double dt = D.scal[SC_DT];
+    const double beta_old = dt / (2.0 * tau_of_w(D.gas, W)), beta = dt / (2.0 * tau_of_w(D.gas, Wn));
+    const size_t cb0 = (size_t)c * nvel;
+    // pass 1: new targets, kappa = heat(Wn, K), heat(Wn, M), rold = heat(W, h - mh, b - mb)
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double akxh, akxb, akyh, akyb;
+        target_kernels(node, ck, u, v, &akxh, &akxb, &akyh, &akyb);
+        const size_t o = cb0 + k;
+        D.tgn[0][o] = amh[k]; D.tgn[1][o] = amb[k];
+        D.tgn[2][o] = akxh; D.tgn[3][o] = akxb; D.tgn[4][o] = akyh; D.tgn[5][o] = akyb;
+        const double cx = u - ux, cy = v - vy;
+        heat_add(red + 0, w, cx, cy, akxh, akxb);
+        heat_add(red + 2, w, cx, cy, akyh, akyb);
+        heat_add(red + 4, w, cx, cy, amh[k], amb[k]);
+        heat_add(red + 6, w, u - ux_o, v - vy_o, D.h[o] - D.tg[0][o], D.b[o] - D.tg[1][o]);
+    }
+    block_sum<8>(red, lds);
+    const double kap[4] = {red[0], red[1], red[2], red[3]};   // kappa(:,1) = heat(kx), kappa(:,2) = heat(ky)
+    const double qm[2] = {red[4], red[5]}, rold[2] = {red[6], red[7]};
+    // pass 2: heat(Wn, rh, rb), rh = hs + beta_old (mh + kxh rold1 + kyh rold2 - h)
+    double rq[2] = {0.0, 0.0};
+    UGKS_VLOOP(k, NT, nvel) {
+        const size_t o = cb0 + k;
+        const double rhk = D.rh[o] + beta_old * (D.tg[0][o] + D.tg[2][o] * rold[0] + D.tg[4][o] * rold[1] - D.h[o]);
+        const double rbk = D.rb[o] + beta_old * (D.tg[1][o] + D.tg[3][o] * rold[0] + D.tg[5][o] * rold[1] - D.b[o]);
+        heat_add(rq, D.w[k], D.u[k] - ux, D.v[k] - vy, rhk, rbk);
+    }
+    block_sum<2>(rq, lds);
+    const double rhs0 = rq[0] - qm[0], rhs1 = rq[1] - qm[1];
+    const double a11 = 1.0 + beta - beta * kap[0], a21 = -beta * kap[1];
+    const double a12 = -beta * kap[2], a22 = 1.0 + beta - beta * kap[3];
+    const double det = a11 * a22 - a12 * a21;
+    if (!isfinite(det) || !(det > 0.0)) {
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_SOLVE | ST_AT_CELL));
+        return;
+    }
+    const double rnew0 = (rhs0 * a22 - a12 * rhs1) / det, rnew1 = (a11 * rhs1 - rhs0 * a21) / det;
+    // pass 3: hn, bn and the gate quantities
+    double mx[6] = {0.0, 1.0, -1e308, -1e308, -1e308, -1e308};   // e1, scale, -min h, max h, -min b, max b
+    double sm[6] = {0, 0, 0, 0, 0, 0};                          // moments of hn, bn (4), rhat (2)
+    int fin = 1;
+    UGKS_VLOOP(k, NT, nvel) {
+        const size_t o = cb0 + k;
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        const double rhk = D.rh[o] + beta_old * (D.tg[0][o] + D.tg[2][o] * rold[0] + D.tg[4][o] * rold[1] - D.h[o]);
+        const double rbk = D.rb[o] + beta_old * (D.tg[1][o] + D.tg[3][o] * rold[0] + D.tg[5][o] * rold[1] - D.b[o]);
+        const double eqh = amh[k] + D.tgn[2][o] * rnew0 + D.tgn[4][o] * rnew1;
+        const double eqb = amb[k] + D.tgn[3][o] * rnew0 + D.tgn[5][o] * rnew1;
+        const double hnk = (rhk + beta * eqh) / (1.0 + beta), bnk = (rbk + beta * eqb) / (1.0 + beta);
+        D.hn[o] = hnk;
+        D.bn[o] = bnk;
+        fin &= (isfinite(hnk) && isfinite(bnk)) ? 1 : 0;
+        mx[0] = fmax(mx[0], fmax(fabs((1 + beta) * hnk - rhk - beta * eqh), fabs((1 + beta) * bnk - rbk - beta * eqb)));
+        mx[1] = fmax(mx[1], fmax(fmax(fabs(rhk), fabs(rbk)), fmax(fabs(hnk), fabs(bnk))));
+        mx[2] = fmax(mx[2], -hnk); mx[3] = fmax(mx[3], hnk);
+        mx[4] = fmax(mx[4], -bnk); mx[5] = fmax(mx[5], bnk);
+        mom_add(sm, w, u, v, hnk, bnk);
+        heat_add(sm + 4, w, u - ux, v - vy, hnk - amh[k], bnk - amb[k]);
+    }
+    block_max<6>(mx, lds);
+    block_sum<6>(sm, lds);
+    if (!__syncthreads_and(fin)) {
+        if (threadIdx.x == 0) atomicOr(D.status, (unsigned)(ST_NONFINITE | ST_AT_CELL));
+        return;
+    }
+    if (threadIdx.x == 0) {
+        double cn = mx[0] / mx[1];
+        double rs = fmax(1.0, fmax(fmax(fabs(sm[4]), fabs(sm[5])), fmax(fabs(rnew0), fabs(rnew1))));
+        cn = fmax(cn, fmax(fabs(sm[4] - rnew0), fabs(sm[5] - rnew1)) / rs);
+        double wm = 1.0, em = 0.0;
+        for (int m = 0; m < 4; ++m) { wm = fmax(wm, fabs(Wn[m])); em = fmax(em, fabs(sm[m] - Wn[m])); }
+        double* cp = D.cnpart + (size_t)c * 8;
+        cp[0] = cn; cp[1] = em / wm; cp[2] = fmax(2.0 * beta_old, 2.0 * beta);
+        cp[3] = -mx[2]; cp[4] = mx[3]; cp[5] = -mx[4]; cp[6] = mx[5];
+    }
+}
+
+// gates of the step and the conservation ledger (one workgroup)
+__global__ void __launch_bounds__(256) k_cn_finish(Dev D, double umax, double vmax) {
+    __shared__ double lds[16 * 9];
+    const int nc = D.ncell, nx = D.nx, ny = D.ny;
+    double mx[7] = {0.0, 0.0, 0.0, -1e308, -1e308, -1e308, -1e308};   // cn, mm, 2beta, -hmin, hmax, -bmin, bmax
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        const double* cp = D.cnpart + (size_t)c * 8;
+        mx[0] = fmax(mx[0], cp[0]); mx[1] = fmax(mx[1], cp[1]); mx[2] = fmax(mx[2], cp[2]);
+        mx[3] = fmax(mx[3], -cp[3]); mx[4] = fmax(mx[4], cp[4]); mx[5] = fmax(mx[5], -cp[5]); mx[6] = fmax(mx[6], cp[6]);
+    }
+    block_max<7>(mx, lds);
+    double tn[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // totals of Wn and of W
+    for (int c = threadIdx.x; c < nc; c += blockDim.x)
+        for (int m = 0; m < 4; ++m) { tn[m] += D.Wn[4 * (size_t)c + m]; tn[4 + m] += D.W[4 * (size_t)c + m]; }
+    block_sum<8>(tn, lds);
+    double wl[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // bottom, top wall fluxes / dy
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        for (int m = 0; m < 4; ++m) {
+            wl[m] += D.yfw[4 * (size_t)i + m] / D.dy;
+            wl[4 + m] += D.yfw[4 * ((size_t)ny * nx + i) + m] / D.dy;
+        }
+    block_sum<8>(wl, lds);
+    if (threadIdx.x != 0) return;
+    double* S = D.scal;
+    if (S[SC_CN_INIT] == 0.0) {
+        for (int m = 0; m < 4; ++m) { S[SC_CN_TOTAL0 + m] = tn[4 + m]; S[SC_CN_BAL + m] = 0.0; }
+        S[SC_CN_INIT] = 1.0;
+    }
+    double* dg = S + SC_CN_DIAG;
+    for (int q = 0; q < 16; ++q) dg[q] = 0.0;
+    dg[2] = mx[0]; dg[3] = mx[1]; dg[6] = mx[2];
+    dg[7] = S[SC_DT] * (umax / D.dx + vmax / D.dy);
+    dg[8] = -mx[3]; dg[9] = mx[4]; dg[10] = -mx[5]; dg[11] = mx[6]; dg[13] = S[SC_DT];
+    S[SC_CN_ACCEPT] = 0.0;
+    if (!(mx[0] <= UGKS_GATE) || !(mx[1] <= UGKS_GATE)) { dg[14] = 1.0; return; }
+    double drift = 0.0, next[4];
+    for (int m = 0; m < 4; ++m) {
+        next[m] = S[SC_CN_BAL + m] + wl[4 + m] - wl[m];
+        drift = fmax(drift, fabs(tn[m] + next[m] - S[SC_CN_TOTAL0 + m]) / fmax(1.0, fabs(S[SC_CN_TOTAL0 + m])));
+    }
+    dg[12] = drift;
+    if (!(drift <= UGKS_GATE)) { dg[15] = 1.0; return; }
+    for (int m = 0; m < 4; ++m) S[SC_CN_BAL + m] = next[m];
+    S[SC_CN_ACCEPT] = 1.0;
 }
 
 // ===========================================================================
@@ -1060,11 +1283,49 @@ static hipError_t launch_closure_kernels(const Dev& D, hipStream_t s, Prof* P) {
     return hipSuccess;
 }
 
+template <int NT>
+static hipError_t launch_cn(const Dev& D, bool build_old, double umax, double vmax, hipStream_t s, Prof* P) {
+    static size_t lds_t = 0, lds_c = 0, lds_face = 0, lds_wall = 0;
+    const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
+    hipError_t e;
+    if ((e = allow_lds(k_cn_targets<NT>, q2, &lds_t)) != hipSuccess) return e;
+    if ((e = allow_lds(k_cn_cell<NT>, q2, &lds_c)) != hipSuccess) return e;
+    if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
+    if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
+    const dim3 gw((4 * D.nx + 255) / 256, D.ny);
+    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, D, D.W, D.ws, D.wv));
+    if (build_old) UGKS_TIMED(P, K_CN_TARGETS, s, hipLaunchKernelGGL(k_cn_targets<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
+    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell), dim3(128), 0, s, D));
+    UGKS_TIMED(P, K_FACE_FLUX, s,
+               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), q1, s, D));
+    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx), dim3(NT), q1, s, D));
+    UGKS_TIMED(P, K_CN_STREAM, s,
+               hipLaunchKernelGGL(k_cn_stream, dim3((D.nvel + 255) / 256, D.ncell), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, D, D.Wn, D.ws, D.wv));
+    UGKS_TIMED(P, K_CN_CELL, s, hipLaunchKernelGGL(k_cn_cell<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
+    UGKS_TIMED(P, K_CN_FINISH, s, hipLaunchKernelGGL(k_cn_finish, dim3(1), dim3(256), 0, s, D, umax, vmax));
+    return hipGetLastError();
+}
+
+// One S2CN2 step on the current state (build_old: first build the volume
+// targets of the state itself).  The proposal goes to Wn/hn/bn and tgn.
+extern "C" hipError_t ugks_launch_cn(const Dev* Dp, int build_old, double umax, double vmax, hipStream_t s,
+                                     Prof* P) {
+    switch (Dp->wg_vel) {
+    case 64: return launch_cn<64>(*Dp, build_old != 0, umax, vmax, s, P);
+    case 128: return launch_cn<128>(*Dp, build_old != 0, umax, vmax, s, P);
+    case 256: return launch_cn<256>(*Dp, build_old != 0, umax, vmax, s, P);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vmax, double residual_tol,
                                            hipStream_t s, Prof* P) {
     const Dev& D = *Dp;
     UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
-    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_WRECON, s,
+               hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D, D.W, D.ws, D.wv));
     UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell), dim3(128), 0, s, D));
     hipError_t e;
     switch (D.wg_vel) {

@@ -17,7 +17,12 @@ enum ScalarSlot {
     SC_REFL_BOT, SC_REFL_TOP,   // reflection normalisers of the frozen wall closure
     SC_NORMS = 16,      // 24 residual diagnostics (layout of ugks_norms)
     SC_WALLS = 40,      // 16 wall fluxes of the last evaluation
-    SC_COUNT = 64
+    SC_CN_INIT = 56,    // S2CN2 conservation ledger initialised
+    SC_CN_ACCEPT,       // last S2CN2 step accepted
+    SC_CN_TOTAL0 = 64,  // 4 conserved totals at the start of the S2CN2 run
+    SC_CN_BAL = 68,     // 4 cumulative wall balance (top - bottom, per unit length / dy)
+    SC_CN_DIAG = 72,    // 16 diagnostics of the last S2CN2 step
+    SC_COUNT = 96
 };
 
 // Upwind face record, rows of nvel values: for each Gauss node a (rows
@@ -57,6 +62,9 @@ struct Dev {
     double *dh, *db;      // forward-sweep micro increments [ncell][nvel]
     double *wallpart;     // [ntiles][2][nx] incidence partials of the wall rows
     double *mompart;      // [ntiles][ncell][4] moment partials of the proposal
+    // S2CN2: volume targets {mh, mb, kxh, kxb, kyh, kyb} of the state and of the proposal
+    double *tg[6], *tgn[6];
+    double *cnpart;       // [ncell][8] per-cell CN diagnostics
     double *scal;         // ScalarSlot
     unsigned *status;
 };
