@@ -61,6 +61,8 @@ def lib():
         L.or_get_norms.argtypes = [C.c_void_p, _D]
         L.or_implicit_update.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
         L.or_cn_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
+        L.or_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.or_explicit_step.argtypes = [C.c_void_p, _D, _D]
         L.or_cn_ledger.argtypes = [C.c_void_p, _D]
         L.or_cn_ledger.restype = None
         L.or_iteration.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
@@ -170,6 +172,19 @@ class OracleSolver:
         d = np.zeros(16)
         st = lib().or_iteration(self.h, C.byref(acc), ptr(d))
         return st, bool(acc.value), d
+
+    def explicit_cold_start(self, rho=1.0, ux=0.0, t_native=None):
+        t = self.case.t_init if t_native is None else t_native
+        st = lib().or_explicit_cold_start(self.h, rho, ux, t)
+        if st:
+            raise RuntimeError(f"oracle explicit cold start failed ({st})")
+
+    def explicit_step(self):
+        """One step of the low-order Explicit entry; returns (status, res4, dt)."""
+        r = np.zeros(4)
+        d = np.zeros(4)
+        st = lib().or_explicit_step(self.h, ptr(r), ptr(d))
+        return st, r, d[0]
 
     def cn_step(self):
         """One S2CN2 explicit step; returns (status, accepted, diag16)."""

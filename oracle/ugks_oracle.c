@@ -2257,3 +2257,472 @@ int or_cn_step(or_solver *s, int *accepted, double *diag) {
 void or_cn_ledger(const or_solver *s, double *out8) {
     for (int m = 0; m < 4; ++m) { out8[m] = s->cn_total0[m]; out8[4 + m] = s->cn_balance[m]; }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Low-order Explicit entry: Van Leer slopes, analytic-moment UGKS flux,     */
+/* multiscale (W2B) diffuse walls, trapezoidal Shakhov update               */
+/* ------------------------------------------------------------------------ */
+
+static const double SMV_D = 1.17549435082228750797e-38;   /* tiny(0.0_real32) */
+
+/* Interp_VanLeer limiter, src/mod_Interpolation.f90:142-171 */
+static double vanleer(double fl, double fc, double fr, double dl, double dr) {
+    double sl = (fc - fl) / dl, sr = (fr - fc) / dr;
+    return (copysign(1.0, sr) + copysign(1.0, sl)) * fabs(sr) * fabs(sl) / (fabs(sr) + fabs(sl) + SMV_D);
+}
+
+/* GetConserved, src/mod_Tools.f90:190-198 (prim = rho, U, V, lambda) */
+static void get_conserved(double gamma, const double prim[4], double w[4]) {
+    w[0] = prim[0];
+    w[1] = prim[0] * prim[1];
+    w[2] = prim[0] * prim[2];
+    w[3] = 0.5 * prim[0] / (prim[3] * (gamma - 1.0)) + 0.5 * prim[0] * (prim[1] * prim[1] + prim[2] * prim[2]);
+}
+
+/* CalcMoment with the half-range moments kept, src/mod_Evolution.f90:1810-1842 */
+static void calc_moment_lr(int ck, const double prim[4], double mu[7], double mul[7], double mur[7], double mv[6],
+                           double mxi[3]) {
+    mul[0] = 0.5 * erfc(-sqrt(prim[3]) * prim[1]);
+    mul[1] = prim[1] * mul[0] + 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(PI_D * prim[3]);
+    mur[0] = 0.5 * erfc(sqrt(prim[3]) * prim[1]);
+    mur[1] = prim[1] * mur[0] - 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(PI_D * prim[3]);
+    for (int i = 2; i <= 6; ++i) {
+        mul[i] = prim[1] * mul[i - 1] + 0.5 * (i - 1) * mul[i - 2] / prim[3];
+        mur[i] = prim[1] * mur[i - 1] + 0.5 * (i - 1) * mur[i - 2] / prim[3];
+    }
+    for (int i = 0; i <= 6; ++i) mu[i] = mul[i] + mur[i];
+    mv[0] = 1.0;
+    mv[1] = prim[2];
+    for (int i = 2; i <= 5; ++i) mv[i] = prim[2] * mv[i - 1] + 0.5 * (i - 1) * mv[i - 2] / prim[3];
+    mxi[0] = 1.0;
+    mxi[1] = 0.5 * ck / prim[3];
+    mxi[2] = ck * (ck + 2.0) / (4.0 * prim[3] * prim[3]);
+}
+
+/* DiscreteMaxwell_LocalVec, src/mod_Tools.f90:44-63 */
+static void maxwell_local(int n, int ck, const double *vn, const double *vt, const double prim[4], double *h,
+                          double *b) {
+    double pre = prim[0] * (prim[3] / PI_D), bf = ck / (2.0 * prim[3]);
+    for (int k = 0; k < n; ++k) {
+        double dvn = vn[k] - prim[1], dvt = vt[k] - prim[2];
+        h[k] = pre * exp(-prim[3] * (dvn * dvn + dvt * dvt));
+        b[k] = h[k] * bf;
+    }
+}
+
+/* GetHeatFlux_LocalVec, src/mod_Tools.f90:382-401 */
+static void heat_local(int n, const double *vn, const double *vt, const double *w, const double *h, const double *b,
+                       const double prim[4], double q[2]) {
+    q[0] = q[1] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double dvn = vn[k] - prim[1], dvt = vt[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
+        q[0] += 0.5 * w[k] * dvn * (v2 * h[k] + b[k]);
+        q[1] += 0.5 * w[k] * dvt * (v2 * h[k] + b[k]);
+    }
+}
+
+/* ShakhovPart_LocalVec (ck > 0), src/mod_Tools.f90:117-153 */
+static void shakhov_local(int n, int ck, double pr, const double *vn, const double *vt, const double *H,
+                          const double *B, const double q[2], const double prim[4], double *hp, double *bp) {
+    double coeff = 0.8 * (1.0 - pr) * prim[3] * prim[3] / prim[0];
+    for (int k = 0; k < n; ++k) {
+        double dvn = vn[k] - prim[1], dvt = vt[k] - prim[2], v2 = dvn * dvn + dvt * dvt;
+        double qdot = dvn * q[0] + dvt * q[1];
+        double shH = 2.0 * prim[3] * v2 - 4.0, shB = shH + 2.0 / ck;
+        hp[k] = coeff * qdot * shH * H[k];
+        bp[k] = coeff * qdot * shB * B[k];
+    }
+}
+
+/* ProjectToConservedMoments (reference = unprojected), src/mod_ImplicitCommon.f90:42-82,
+   velocities (vn, vt) of the frame the target is given in */
+static int project_local(int n, const double *vn, const double *vt, const double *w, double *h, double *b,
+                         const double target[4]) {
+    double mom[4], rhs[4], gram[16] = {0}, lam[4];
+    double sh = 1.0, sb = 1.0;
+    for (int k = 0; k < n; ++k) { if (fabs(h[k]) > sh) sh = fabs(h[k]); if (fabs(b[k]) > sb) sb = fabs(b[k]); }
+    moment_2d(n, h, b, vn, vt, w, mom);
+    for (int a = 0; a < 4; ++a) rhs[a] = target[a] - mom[a];
+    double *wh = (double *)malloc(sizeof(double) * 2 * n), *wb = wh + n;
+    for (int k = 0; k < n; ++k) {
+        double kh[4] = {1.0, vn[k], vt[k], 0.5 * (vn[k] * vn[k] + vt[k] * vt[k])};
+        double kb[4] = {0.0, 0.0, 0.0, 0.5};
+        wh[k] = fabs(h[k]) > DBL_EPSILON * sh ? fabs(h[k]) : DBL_EPSILON * sh;
+        wb[k] = fabs(b[k]) > DBL_EPSILON * sb ? fabs(b[k]) : DBL_EPSILON * sb;
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 4; ++c) gram[c * 4 + a] += w[k] * (wh[k] * kh[a] * kh[c] + wb[k] * kb[a] * kb[c]);
+    }
+    if (!solve_dense4_gj(gram, rhs, lam)) { free(wh); return 0; }
+    for (int k = 0; k < n; ++k) {
+        double kh[4] = {1.0, vn[k], vt[k], 0.5 * (vn[k] * vn[k] + vt[k] * vt[k])};
+        h[k] = h[k] + wh[k] * (kh[0] * lam[0] + kh[1] * lam[1] + kh[2] * lam[2] + kh[3] * lam[3]);
+        b[k] = b[k] + 0.5 * wb[k] * lam[3];
+    }
+    free(wh);
+    moment_2d(n, h, b, vn, vt, w, mom);
+    double rs = 1.0, e = 0.0;
+    for (int a = 0; a < 4; ++a) { if (fabs(target[a]) > rs) rs = fabs(target[a]); if (fabs(mom[a] - target[a]) > e) e = fabs(mom[a] - target[a]); }
+    return e <= 1.0e-10 * rs;
+}
+
+/* PrepareNativeFaceEquilibrium, src/mod_Evolution.f90:16-30 */
+static int face_equilibrium(const or_solver *s, const double *vn, const double *vt, const double prim[4], double *h,
+                            double *b) {
+    double target[4];
+    maxwell_local(s->nvel, s->p.ck, vn, vt, prim, h, b);
+    get_conserved(s->gamma, prim, target);
+    return project_local(s->nvel, vn, vt, s->w, h, b, target);
+}
+
+typedef struct {
+    double *vn, *vt, *h, *b, *shn, *sbn, *sht, *sbt, *H, *B, *Hp, *Bp, *Hw, *Bw;
+} lo_ws;
+
+static void lo_ws_init(lo_ws *w, int n) {
+    double *p = dalloc((size_t)14 * n);
+    w->vn = p; w->vt = p + n; w->h = p + 2 * n; w->b = p + 3 * n; w->shn = p + 4 * n; w->sbn = p + 5 * n;
+    w->sht = p + 6 * n; w->sbt = p + 7 * n; w->H = p + 8 * n; w->B = p + 9 * n; w->Hp = p + 10 * n;
+    w->Bp = p + 11 * n; w->Hw = p + 12 * n; w->Bw = p + 13 * n;
+}
+
+/* Van Leer slopes of every cell, x periodic, one-sided differences on the
+   wall rows (Interpolation with UnifiedPhysicalOperators,
+   src/mod_Interpolation.f90:9-58): slope[(cell)*n + k] */
+static void lo_slopes(const or_solver *s, double *sxh, double *sxb, double *syh, double *syb) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s->ncell; ++c) {
+        int i = c % nx, j = c / nx;
+        size_t cl = (size_t)(j * nx + ip(i - 1, nx)) * n, cr = (size_t)(j * nx + ip(i + 1, nx)) * n,
+               cs = (size_t)(j > 0 ? c - nx : c) * n, cn = (size_t)(j < ny - 1 ? c + nx : c) * n, o = (size_t)c * n;
+        for (int k = 0; k < n; ++k) {
+            sxh[o + k] = vanleer(s->h[cl + k], s->h[o + k], s->h[cr + k], P->dx, P->dx);
+            sxb[o + k] = vanleer(s->b[cl + k], s->b[o + k], s->b[cr + k], P->dx, P->dx);
+            if (j == 0) {
+                syh[o + k] = (s->h[cn + k] - s->h[o + k]) / P->dy;
+                syb[o + k] = (s->b[cn + k] - s->b[o + k]) / P->dy;
+            } else if (j == ny - 1) {
+                syh[o + k] = (s->h[o + k] - s->h[cs + k]) / P->dy;
+                syb[o + k] = (s->b[o + k] - s->b[cs + k]) / P->dy;
+            } else {
+                syh[o + k] = vanleer(s->h[cs + k], s->h[o + k], s->h[cn + k], P->dy, P->dy);
+                syb[o + k] = vanleer(s->b[cs + k], s->b[o + k], s->b[cn + k], P->dy, P->dy);
+            }
+        }
+    }
+}
+
+/* CalcFlux_MultiDimension (global time step, UnifiedPhysicalOperators),
+   src/mod_Evolution.f90:186-492.  Interior face between cells L and R;
+   idx 1: x face (normal +x), 2: y face (normal +y).  Writes the micro fluxes
+   (x length x dt) and the macro flux = their discrete moments. */
+static int lo_face_flux(const or_solver *s, lo_ws *ws, int idx, int cL, int cR, const double *sxh, const double *sxb,
+                        const double *syh, const double *syb, double dt, double *fh, double *fb, double fw[4]) {
+    const or_params *P = &s->p;
+    int n = s->nvel, ck = P->ck;
+    double cosx = idx == 1 ? 1.0 : 0.0, cosy = idx == 1 ? 0.0 : 1.0;
+    double len_n = idx == 1 ? P->dx : P->dy, flen = idx == 1 ? P->dy : P->dx;
+    const double *snh = idx == 1 ? sxh : syh, *snb = idx == 1 ? sxb : syb;
+    const double *sth = idx == 1 ? syh : sxh, *stb = idx == 1 ? syb : sxb;
+    double tsg = idx == 1 ? 1.0 : -1.0;
+    size_t oL = (size_t)cL * n, oR = (size_t)cR * n;
+    double wL[4], wR[4];
+    const double *WL = s->W + 4 * cL, *WR = s->W + 4 * cR;
+    /* LocalFrame */
+    wL[0] = WL[0]; wL[1] = WL[1] * cosx + WL[2] * cosy; wL[2] = -WL[1] * cosy + WL[2] * cosx; wL[3] = WL[3];
+    wR[0] = WR[0]; wR[1] = WR[1] * cosx + WR[2] * cosy; wR[2] = -WR[1] * cosy + WR[2] * cosx; wR[3] = WR[3];
+    double cv[4] = {0, 0, 0, 0}, swt[4] = {0, 0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        double vn = s->u[k] * cosx + s->v[k] * cosy, vt = -s->u[k] * cosy + s->v[k] * cosx;
+        double dL = (copysign(1.0, vn) + 1.0) / 2.0, dR = 1.0 - dL;
+        ws->vn[k] = vn; ws->vt[k] = vt;
+        ws->h[k] = (s->h[oL + k] + 0.5 * len_n * snh[oL + k]) * dL + (s->h[oR + k] - 0.5 * len_n * snh[oR + k]) * dR;
+        ws->b[k] = (s->b[oL + k] + 0.5 * len_n * snb[oL + k]) * dL + (s->b[oR + k] - 0.5 * len_n * snb[oR + k]) * dR;
+        ws->shn[k] = snh[oL + k] * dL + snh[oR + k] * dR;
+        ws->sbn[k] = snb[oL + k] * dL + snb[oR + k] * dR;
+        ws->sht[k] = tsg * (sth[oL + k] * dL + sth[oR + k] * dR);
+        ws->sbt[k] = tsg * (stb[oL + k] * dL + stb[oR + k] * dR);
+        double v2 = vn * vn + vt * vt, wk = s->w[k];
+        cv[0] += wk * ws->h[k]; cv[1] += wk * vn * ws->h[k]; cv[2] += wk * vt * ws->h[k];
+        cv[3] += 0.5 * wk * (v2 * ws->h[k] + ws->b[k]);
+        swt[0] += wk * ws->sht[k]; swt[1] += wk * vn * ws->sht[k]; swt[2] += wk * vt * ws->sht[k];
+        swt[3] += 0.5 * wk * (v2 * ws->sht[k] + ws->sbt[k]);
+    }
+    if (!is_physical(cv)) return OR_NONFINITE;
+    double prim[4], sw[4], aL[4], aR[4], bs[4], aT[4];
+    get_primary(s->gamma, cv, prim);
+    for (int m = 0; m < 4; ++m) sw[m] = (cv[m] - wL[m]) / (0.5 * len_n);
+    micro_slope(ck, prim, sw, aL);
+    for (int m = 0; m < 4; ++m) sw[m] = (wR[m] - cv[m]) / (0.5 * len_n);
+    micro_slope(ck, prim, sw, aR);
+    micro_slope(ck, prim, swt, bs);
+    double mu[7], mul[7], mur[7], mv[6], mxi[3], mauL[4], mauR[4], mbv[4];
+    calc_moment_lr(ck, prim, mu, mul, mur, mv, mxi);
+    moment_auvxi(aL, mul, mv, mxi, 1, 0, mauL);
+    moment_auvxi(aR, mur, mv, mxi, 1, 0, mauR);
+    moment_auvxi(bs, mu, mv, mxi, 0, 1, mbv);
+    for (int m = 0; m < 4; ++m) sw[m] = -prim[0] * (mauL[m] + mauR[m] + mbv[m]);
+    micro_slope(ck, prim, sw, aT);
+    double tau = get_tau(P, prim), ex = exp(-dt / tau), Mt[5];
+    Mt[3] = tau * (1.0 - ex);
+    Mt[4] = -tau * dt * ex + tau * Mt[3];
+    Mt[0] = dt - Mt[3];
+    Mt[1] = -tau * Mt[0] + Mt[4];
+    Mt[2] = 0.5 * dt * dt - tau * Mt[0];
+    if (!face_equilibrium(s, ws->vn, ws->vt, prim, ws->H, ws->B)) return OR_GATE;
+    double qf[2];
+    heat_local(n, ws->vn, ws->vt, s->w, ws->h, ws->b, prim, qf);
+    shakhov_local(n, ck, P->pr, ws->vn, ws->vt, ws->H, ws->B, qf, prim, ws->Hp, ws->Bp);
+    for (int k = 0; k < n; ++k) {
+        double vn = ws->vn[k], vt = ws->vt[k], v2 = vn * vn + vt * vt;
+        double dL = (copysign(1.0, vn) + 1.0) / 2.0, dR = 1.0 - dL;
+        double slh = vn * ws->shn[k] + vt * ws->sht[k], slb = vn * ws->sbn[k] + vt * ws->sbt[k];
+        double H0 = ws->H[k], B0 = ws->B[k];
+#define LO_EXP(a, X, Y) ((a)[0] * (X) + (a)[1] * vn * (X) + (a)[2] * vt * (X) + 0.5 * (a)[3] * (v2 * (X) + (Y)))
+        double h_aL = LO_EXP(aL, H0, B0), h_aR = LO_EXP(aR, H0, B0), h_b = LO_EXP(bs, H0, B0), h_aT = LO_EXP(aT, H0, B0);
+        double b_aL = LO_EXP(aL, B0, mxi[2] * H0), b_aR = LO_EXP(aR, B0, mxi[2] * H0), b_b = LO_EXP(bs, B0, mxi[2] * H0),
+               b_aT = LO_EXP(aT, B0, mxi[2] * H0);
+#undef LO_EXP
+        double F_h = Mt[0] * vn * (H0 + ws->Hp[k]) + Mt[1] * vn * vn * (h_aL * dL + h_aR * dR) + Mt[1] * vn * vt * h_b +
+                     Mt[2] * vn * h_aT + Mt[3] * vn * ws->h[k] - Mt[4] * vn * slh;
+        double F_b = Mt[0] * vn * (B0 + ws->Bp[k]) + Mt[1] * vn * vn * (b_aL * dL + b_aR * dR) + Mt[1] * vn * vt * b_b +
+                     Mt[2] * vn * b_aT + Mt[3] * vn * ws->b[k] - Mt[4] * vn * slb;
+        fh[k] = flen * F_h;
+        fb[k] = flen * F_b;
+    }
+    moment_2d(n, fh, fb, s->u, s->v, s->w, fw);
+    return OR_OK;
+}
+
+/* W2BMultiscaleFluxBoundary (UnifiedPhysicalOperators), src/mod_Evolution.f90:1888-2449,
+   on the y walls: side -1 bottom (face of row 0), +1 top (face of row ny-1). */
+static int lo_wall_flux(const or_solver *s, lo_ws *ws, int side, int i, const double *sxh, const double *sxb,
+                        const double *syh, const double *syb, double dt, double *fh, double *fb, double fw[4]) {
+    const or_params *P = &s->p;
+    int n = s->nvel, ck = P->ck, nx = P->nx;
+    int c = side < 0 ? i : (P->ny - 1) * nx + i;
+    size_t o = (size_t)c * n;
+    double rot = -(double)side, flen = P->dx, len = P->dy;
+    const double *W = s->W + 4 * c;
+    for (int k = 0; k < n; ++k) {
+        double vn = s->v[k], vt = -s->u[k];
+        ws->vn[k] = vn; ws->vt[k] = vt;
+        ws->shn[k] = syh[o + k]; ws->sbn[k] = syb[o + k];
+        ws->sht[k] = -sxh[o + k]; ws->sbt[k] = -sxb[o + k];
+        ws->h[k] = s->h[o + k] - rot * 0.5 * len * ws->shn[k];
+        ws->b[k] = s->b[o + k] - rot * 0.5 * len * ws->sbn[k];
+    }
+    /* WallPrimary in the wall frame: (rho, 0, -U_wall, lambda_wall) */
+    double lam_w = 1.0 / P->wall_tn[side < 0 ? 0 : 1];
+    double prim_w[4] = {1.0, 0.0, -P->wall_ux[side < 0 ? 0 : 1], lam_w};
+    double pc[4], prim[4], prim_g[4];
+    get_primary(s->gamma, W, pc);
+    prim[0] = pc[0]; prim[1] = pc[2]; prim[2] = -pc[1]; prim[3] = pc[3];   /* LocalFrame of a y face */
+    prim_g[0] = prim[0] / prim[3] * prim_w[3];
+    prim_g[1] = prim_w[1]; prim_g[2] = prim_w[2]; prim_g[3] = prim_w[3];
+    double tau = get_tau(P, prim_g), ex = exp(-dt / tau), Mt[5];
+    Mt[3] = tau * (1.0 - ex);
+    Mt[4] = -tau * dt * ex + tau * Mt[3];
+    Mt[0] = dt - Mt[3];
+    Mt[1] = -tau * Mt[0] + Mt[4];
+    Mt[2] = 0.5 * dt * dt - tau * Mt[0];
+    if (!face_equilibrium(s, ws->vn, ws->vt, prim_g, ws->H, ws->B)) return OR_GATE;
+    double qf[2];
+    heat_local(n, ws->vn, ws->vt, s->w, ws->h, ws->b, prim_g, qf);
+    shakhov_local(n, ck, P->pr, ws->vn, ws->vt, ws->H, ws->B, qf, prim_g, ws->Hp, ws->Bp);
+    double cg[4], cg1[4], sw[4], a_gn[4], a_gt[4], a_T[4], swt[4] = {0, 0, 0, 0};
+    get_conserved(s->gamma, prim_g, cg);
+    cg1[0] = W[0]; cg1[1] = W[2]; cg1[2] = -W[1]; cg1[3] = W[3];
+    for (int m = 0; m < 4; ++m) sw[m] = (cg[m] - cg1[m]) / ((double)side * 0.5 * len);
+    micro_slope(ck, prim_g, sw, a_gn);
+    for (int k = 0; k < n; ++k) {
+        double vn = ws->vn[k], vt = ws->vt[k], v2 = vn * vn + vt * vt, wk = s->w[k];
+        swt[0] += wk * ws->sht[k]; swt[1] += wk * vn * ws->sht[k]; swt[2] += wk * vt * ws->sht[k];
+        swt[3] += 0.5 * wk * (v2 * ws->sht[k] + ws->sbt[k]);
+    }
+    micro_slope(ck, prim_g, swt, a_gt);
+    double mu[7], mul[7], mur[7], mv[6], mxi[3], mau[4], mbv[4];
+    calc_moment_lr(ck, prim_g, mu, mul, mur, mv, mxi);
+    moment_auvxi(a_gn, mu, mv, mxi, 1, 0, mau);
+    moment_auvxi(a_gt, mu, mv, mxi, 0, 1, mbv);
+    for (int m = 0; m < 4; ++m) sw[m] = -prim_g[0] * (mau[m] + mbv[m]);
+    micro_slope(ck, prim_g, sw, a_T);
+    double prim_shape[4] = {1.0, prim_w[1], prim_w[2], prim_w[3]};
+    if (!face_equilibrium(s, ws->vn, ws->vt, prim_shape, ws->Hw, ws->Bw)) return OR_GATE;
+    double incidence = 0.0, reflection = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double vn = ws->vn[k], vt = ws->vt[k], v2 = vn * vn + vt * vt, wk = s->w[k];
+        double delta = (copysign(1.0, vn) * rot + 1.0) / 2.0, inflow = 1.0 - delta;
+        double slh = vn * ws->shn[k] + vt * ws->sht[k];
+        double Hg = ws->H[k], Bg = ws->B[k];
+        double Ha_gn = (a_gn[0] + a_gn[1] * vn + a_gn[2] * vt) * Hg + 0.5 * a_gn[3] * (v2 * Hg + Bg);
+        double Ha_gt = (a_gt[0] + a_gt[1] * vn + a_gt[2] * vt) * Hg + 0.5 * a_gt[3] * (v2 * Hg + Bg);
+        double Ha_T = (a_T[0] + a_T[1] * vn + a_T[2] * vt) * Hg + 0.5 * a_T[3] * (v2 * Hg + Bg);
+        incidence += wk * vn * inflow *
+                     (Mt[0] * (Hg + ws->Hp[k]) + Mt[1] * (vn * Ha_gn + vt * Ha_gt) + Mt[2] * Ha_T + Mt[3] * ws->h[k] -
+                      Mt[4] * slh);
+        reflection += wk * vn * ws->Hw[k] * delta;
+    }
+    reflection = dt * reflection;
+    double rho_w = -incidence / reflection;
+    if (!isfinite(rho_w)) return OR_GATE;
+    double loc[4] = {0, 0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        double vn = ws->vn[k], vt = ws->vt[k], v2 = vn * vn + vt * vt;
+        double delta = (copysign(1.0, vn) * rot + 1.0) / 2.0, inflow = 1.0 - delta;
+        double Hg = ws->H[k], Bg = ws->B[k], Hw = rho_w * ws->Hw[k], Bw = rho_w * ws->Bw[k];
+        double Ha_gn = (a_gn[0] + a_gn[1] * vn + a_gn[2] * vt) * Hg + 0.5 * a_gn[3] * (v2 * Hg + Bg);
+        double Ha_gt = (a_gt[0] + a_gt[1] * vn + a_gt[2] * vt) * Hg + 0.5 * a_gt[3] * (v2 * Hg + Bg);
+        double Ha_T = (a_T[0] + a_T[1] * vn + a_T[2] * vt) * Hg + 0.5 * a_T[3] * (v2 * Hg + Bg);
+        double Ba_gn = (a_gn[0] + a_gn[1] * vn + a_gn[2] * vt) * Bg + 0.5 * a_gn[3] * (v2 * Bg + mxi[2] * Hg);
+        double Ba_gt = (a_gt[0] + a_gt[1] * vn + a_gt[2] * vt) * Bg + 0.5 * a_gt[3] * (v2 * Bg + mxi[2] * Hg);
+        double Ba_T = (a_T[0] + a_T[1] * vn + a_T[2] * vt) * Bg + 0.5 * a_T[3] * (v2 * Bg + mxi[2] * Hg);
+        fh[k] = flen * (dt * vn * Hw * delta + Mt[0] * vn * (Hg + ws->Hp[k]) * inflow + Mt[1] * vn * vn * Ha_gn * inflow +
+                        Mt[1] * vn * vt * Ha_gt * inflow + Mt[2] * vn * Ha_T * inflow + Mt[3] * vn * ws->h[k] * inflow -
+                        Mt[4] * vn * (vn * ws->shn[k] + vt * ws->sht[k]) * inflow);
+        fb[k] = flen * (dt * vn * Bw * delta + Mt[0] * vn * (Bg + ws->Bp[k]) * inflow + Mt[1] * vn * vn * Ba_gn * inflow +
+                        Mt[1] * vn * vt * Ba_gt * inflow + Mt[2] * vn * Ba_T * inflow + Mt[3] * vn * ws->b[k] * inflow -
+                        Mt[4] * vn * (vn * ws->sbn[k] + vt * ws->sbt[k]) * inflow);
+        double hk = fh[k] / flen, bk = fb[k] / flen, wk = s->w[k];
+        loc[0] += wk * hk; loc[1] += wk * vn * hk; loc[2] += wk * vt * hk; loc[3] += wk * 0.5 * (v2 * hk + bk);
+    }
+    /* GlobalFrame of a y face, times the face length */
+    fw[0] = flen * loc[0]; fw[1] = flen * (-loc[2]); fw[2] = flen * loc[1]; fw[3] = flen * loc[3];
+    return OR_OK;
+}
+
+/* Freestream cold start of the Explicit entry (InitFlow_Freestream,
+   src/mod_InitializeFlow.f90:826-856): W = GetConserved(prim), f = the
+   analytic Maxwellian at the velocity points (DiscreteMaxwell_Vec), then
+   projected onto W (EnforceAcceptedMomentCompatibility, called for
+   UnifiedPhysicalOperators by InitializeSteadyImplicit,
+   src/mod_InitializeFlow.f90:44, src/mod_ImplicitMicro.f90:308-323). */
+int or_explicit_cold_start(or_solver *s, double rho, double ux, double tnative) {
+    int n = s->nvel;
+    double prim[4] = {rho, ux, 0.0, 1.0 / tnative}, w[4];
+    get_conserved(s->gamma, prim, w);
+    double q[2] = {0.0, 0.0};
+    double *h = dalloc(n), *b = dalloc(n);
+    maxwell_shakhov(s, prim, q, h, b);   /* q = 0: the plain Maxwellian */
+    if (!project_moments(s, h, b, w)) { free(h); free(b); return OR_GATE; }
+    for (int c = 0; c < s->ncell; ++c) {
+        memcpy(s->W + 4 * c, w, sizeof w);
+        memcpy(s->h + (size_t)c * n, h, sizeof(double) * n);
+        memcpy(s->b + (size_t)c * n, b, sizeof(double) * n);
+    }
+    free(h); free(b);
+    return OR_OK;
+}
+
+/* One step of the Explicit entry (TimeStep, Interpolation, EvolutionNative
+   with the W2B walls, ComputeResidual, ApplyExplicitUpdate), src/mod_Solve.f90:84-165,
+   src/mod_TimeIntegrator.f90:16-134,235-293.  res4: the residual res(1:4)
+   Write_residuals prints; diag4: [dt, 0, 0, 0]. */
+int or_explicit_step(or_solver *s, double *res4, double *diag4) {
+    const or_params *P = &s->p;
+    int nx = P->nx, ny = P->ny, n = s->nvel, nc = s->ncell, ck = P->ck;
+    double dt = time_step(s);
+    s->dt = dt;
+    size_t cv = (size_t)nc * n;
+    double *sxh = dalloc(cv), *sxb = dalloc(cv), *syh = dalloc(cv), *syb = dalloc(cv);
+    lo_slopes(s, sxh, sxb, syh, syb);
+    /* face fluxes (x length x dt): x faces = east face of each cell, y faces = south face of row j (0..ny) */
+    double *xfh = dalloc(cv), *xfb = dalloc(cv), *yfh = dalloc((size_t)(ny + 1) * nx * n),
+           *yfb = dalloc((size_t)(ny + 1) * nx * n), *xfw = dalloc((size_t)nc * 4), *yfw = dalloc((size_t)(ny + 1) * nx * 4);
+    int status = OR_OK;
+#pragma omp parallel
+    {
+        lo_ws ws;
+        lo_ws_init(&ws, n);
+        int my = OR_OK;
+#pragma omp for schedule(dynamic, 4)
+        for (int f = 0; f < nc + (ny - 1) * nx + 2 * nx; ++f) {
+            if (my != OR_OK) continue;
+            if (f < nc) {
+                int j = f / nx, i = f % nx;
+                my = lo_face_flux(s, &ws, 1, f, j * nx + ip(i + 1, nx), sxh, sxb, syh, syb, dt, xfh + (size_t)f * n,
+                                  xfb + (size_t)f * n, xfw + 4 * (size_t)f);
+            } else if (f < nc + (ny - 1) * nx) {
+                int g = f - nc;   /* between rows j and j+1 */
+                size_t fo = (size_t)g + nx;
+                my = lo_face_flux(s, &ws, 2, g, g + nx, sxh, sxb, syh, syb, dt, yfh + fo * n, yfb + fo * n, yfw + 4 * fo);
+            } else {
+                int q = f - nc - (ny - 1) * nx, side = q < nx ? -1 : 1, i = q % nx;
+                size_t fo = (size_t)(side < 0 ? 0 : ny) * nx + i;
+                my = lo_wall_flux(s, &ws, side, i, sxh, sxb, syh, syb, dt, yfh + fo * n, yfb + fo * n, yfw + 4 * fo);
+            }
+        }
+#pragma omp critical
+        if (my != OR_OK && status == OR_OK) status = my;
+        free(ws.vn);
+    }
+    double sumRes[4] = {0, 0, 0, 0}, sumAvg[4] = {0, 0, 0, 0}, areaSq = 0.0;
+    const double area = P->dx * P->dy, ainv = 1.0 / area;
+    if (status == OR_OK) {
+        double *Wn = dalloc((size_t)nc * 4);
+#pragma omp parallel
+        {
+            double *Ho = dalloc(n), *Bo = dalloc(n), *H = dalloc(n), *B = dalloc(n);
+            int my = OR_OK;
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                if (my != OR_OK) continue;
+                int j = c / nx, i = c % nx, cw = j * nx + ip(i - 1, nx);
+                size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i, o = (size_t)c * n;
+                const double *Wo = s->W + 4 * c;
+                double po[4], pn[4], wn[4], q[2];
+                get_primary(s->gamma, Wo, po);
+                double tau_o = get_tau(P, po);
+                for (int m = 0; m < 4; ++m)
+                    wn[m] = Wo[m] + (xfw[4 * (size_t)cw + m] - xfw[4 * (size_t)c + m] + yfw[4 * fs + m] - yfw[4 * fn + m]) / area;
+                if (!is_physical(wn)) { my = OR_NONFINITE; continue; }
+                get_primary(s->gamma, wn, pn);
+                double tau = get_tau(P, pn);
+                memcpy(Wn + 4 * c, wn, sizeof wn);
+                heat_flux_vec(s, s->h + o, s->b + o, po, q);
+                maxwell_shakhov(s, po, q, Ho, Bo);
+                maxwell_shakhov(s, pn, q, H, B);
+                double wo_c[4];
+                get_conserved(s->gamma, po, wo_c);
+                if (!project_moments(s, Ho, Bo, wo_c) || !project_moments(s, H, B, wn)) { my = OR_GATE; continue; }
+                double inv = 1.0 / (1.0 + 0.5 * dt / tau);
+                for (int k = 0; k < n; ++k) {
+                    double ho = s->h[o + k], bo = s->b[o + k];
+                    double rh = (xfh[(size_t)cw * n + k] - xfh[o + k] + yfh[fs * n + k] - yfh[fn * n + k]) * ainv;
+                    double rb = (xfb[(size_t)cw * n + k] - xfb[o + k] + yfb[fs * n + k] - yfb[fn * n + k]) * ainv;
+                    s->h[o + k] = (ho + rh + 0.5 * dt * (H[k] / tau + (Ho[k] - ho) / tau_o)) * inv;
+                    s->b[o + k] = (bo + rb + 0.5 * dt * (B[k] / tau + (Bo[k] - bo) / tau_o)) * inv;
+                }
+            }
+#pragma omp critical
+            if (my != OR_OK && status == OR_OK) status = my;
+            free(Ho); free(Bo); free(H); free(B);
+        }
+        if (status == OR_OK) {
+            for (int c = 0; c < nc; ++c) {
+                for (int m = 0; m < 4; ++m) {
+                    double d = (s->W[4 * c + m] - Wn[4 * c + m]) * area / dt;
+                    sumRes[m] += d * d;
+                    sumAvg[m] += fabs(Wn[4 * c + m]) * area;
+                }
+                areaSq += area * area;
+            }
+            memcpy(s->W, Wn, sizeof(double) * nc * 4);
+        }
+        free(Wn);
+    }
+    (void)ck;
+    if (res4) {
+        double ord = fmax(fmax(sumAvg[0], sumAvg[3]), SMV_D);
+        for (int m = 0; m < 4; ++m) {
+            double num = sqrt((double)(nx * ny) * sumRes[m]), den = sumAvg[m] + SMV_D;
+            double rel = num / den, ab = sqrt(sumRes[m] / fmax(areaSq, SMV_D));
+            res4[m] = sumAvg[m] <= sqrt(DBL_EPSILON) * fmax(ord, DBL_MIN) ? ab : rel;
+        }
+    }
+    if (diag4) { diag4[0] = dt; diag4[1] = diag4[2] = diag4[3] = 0.0; }
+    free(sxh); free(sxb); free(syh); free(syb);
+    free(xfh); free(xfb); free(yfh); free(yfb); free(xfw); free(yfw);
+    return status;
+}

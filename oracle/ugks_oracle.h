@@ -80,6 +80,11 @@ int or_cn_step(or_solver *s, int *accepted, double *diag16);
 /* S2CN2 conservation ledger: total0[4] then wall balance[4] */
 void or_cn_ledger(const or_solver *s, double *out8);
 
+/* Low-order Explicit entry: freestream cold start and one step; res4 = the
+   residual res(1:4) the reference prints, diag4[0] = dt */
+int or_explicit_cold_start(or_solver *s, double rho, double ux, double tnative);
+int or_explicit_step(or_solver *s, double *res4, double *diag4);
+
 #ifdef __cplusplus
 }
 #endif

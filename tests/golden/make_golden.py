@@ -5,6 +5,10 @@ Run once in a container that holds the reference checkout (default
 
   couette_release.npz  - the shipped 32x32 velocity table
                          (velocity_half32_scale2.dat: u, v, weight per k)
+  couette_explicit_release.json - the residuals res(1:4) and dtPhysical the
+                         8-rank release run of cases/low_explicit printed
+                         after its second iteration (docs/release_checks/
+                         low_explicit/stdout.log, 7 significant digits)
   couette_release.json - reference-held scalars of the 8-rank release run
                          of cases/high_implicit (Config.ini as shipped):
                          initial raw micro-residual norms reference_h/b
@@ -59,6 +63,20 @@ def main():
     with open(os.path.join(HERE, "couette_release.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", len(rows), "rows; reference_h/b =", out["reference_h"], out["reference_b"])
+
+    ex = {"source": {"stdout": "docs/release_checks/low_explicit/stdout.log",
+                     "case": "cases/low_explicit/Config.ini (TimeIntegration_method Explicit, NP_Y=8; 8 MPI ranks)"}}
+    with open(os.path.join(REF, "docs/release_checks/low_explicit/stdout.log")) as f:
+        for line in f:
+            if "iter,simTime" in line:
+                ex["iteration"] = int(line.split(":")[1].split()[0])
+            elif "dtPhysical:" in line:
+                ex["dt"] = float(line.split(":")[1])
+            elif line.strip().startswith("res:"):
+                ex["res"] = [float(t) for t in line.split(":")[1].split()]
+    with open(os.path.join(HERE, "couette_explicit_release.json"), "w") as f:
+        json.dump(ex, f, indent=1)
+    print("explicit release:", ex)
 
 
 if __name__ == "__main__":
