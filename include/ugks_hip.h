@@ -107,6 +107,23 @@ int ugks_cn_step(void* handle, int* accepted, double* diag16);
  * balance (4) (the reference's total0 and wall_balance). */
 int ugks_cn_ledger(void* handle, double* out8);
 
+/* Low-order Explicit entry (TimeIntegration_method = 'Explicit'): Van Leer
+ * slopes, the analytic-moment UGKS face flux with the projected face
+ * equilibrium and its Shakhov part, the multiscale diffuse walls and the
+ * trapezoidal Shakhov update.  One call is one step of Solve_UGKS's explicit
+ * loop: TimeStep, EvolutionNative, ApplyExplicitUpdate
+ * (src/mod_Solve.f90:84-165,373-421; src/mod_Evolution.f90:46-492,1888-2449;
+ * src/mod_TimeIntegrator.f90:16-134).  res4 (may be NULL): the residual
+ * res(1:4) Write_residuals prints (src/mod_Solve.f90:427-461); dt (may be
+ * NULL): the global time step.  On a failed gate the state is unchanged.
+ * Needs Nu, Nv powers of two up to 64. */
+int ugks_explicit_step(void* handle, double* res4, double* dt);
+/* Freestream start of the Explicit entry: W = GetConserved(rho, ux, 0,
+ * 1/t_native) in every cell and h, b its Maxwellian projected on W
+ * (InitFlow_Freestream, src/mod_InitializeFlow.f90:826-856, with
+ * EnforceAcceptedMomentCompatibility, src/mod_ImplicitMicro.f90:308-323). */
+int ugks_explicit_cold_start(void* handle, double rho, double ux, double t_native);
+
 /* Run n iterations (operator + update each), stopping early on rejection.
  * Returns the number of accepted iterations in *done. */
 int ugks_run(void* handle, int n, int* done);
@@ -138,7 +155,9 @@ int ugks_profile_read(void* handle, int idx, const char** name, double* total_ms
 
 /* Launch-shape tuning (no counterpart in the reference).  Keys:
  *   1  workgroup size of the velocity-parallel operator kernels (face, wall,
- *      cell, cold start): 64, 128 or 256. */
+ *      cell, cold start): 64, 128 or 256;
+ *   2  workgroup size of the Explicit-entry face, wall and cell kernels:
+ *      64, 128, 256 (default), 512 or 1024, at least Nu+Nv and at most Nvel. */
 int ugks_set_tuning(void* handle, int key, int value);
 int ugks_get_tuning(void* handle, int key, int* value);
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 using std::isfinite;
+using std::signbit;
 
 #define __global__
 #define __device__
@@ -41,7 +42,7 @@ struct EmuBlock {
     std::barrier<> bar;
     std::atomic<int> fails{0};
     double xbuf[1024];
-    double rbuf[1024 * 24];
+    double rbuf[1024 * 32];
 };
 
 inline thread_local emu_idx threadIdx, blockIdx;
@@ -140,6 +141,7 @@ inline void emu_launch(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args
 // workgroup reductions: one exchange instead of lane butterflies
 #define UGKS_EMU_REDUCTIONS 1
 inline double lane_xor1(double x) { return __shfl_xor(x, 1, 64); }
+inline double uni(double x) { return x; }
 template <int R>
 inline void block_sum(double* v, double*) {
     EmuBlock& B = *emu_block;

@@ -1,0 +1,103 @@
+"""Low-order Explicit entry of the HIP library (ugks_explicit_step) against
+the oracle.
+
+- Reduced grid (8x16 cells x 16^2 velocities): freestream start and a
+  non-equilibrium start (three HighOrderImplicit oracle iterations), three
+  steps each.  States W, h, b agree to 1e-12 normwise relative and dt to
+  1e-15.  The printed residual res(1:4) agrees to 1e-8 relative (1e-18
+  absolute for the zero mass residual of the first step): res(m) is the norm
+  of W_m(n+1) - W_m(n), a difference of nearly cancelling face fluxes, which
+  turns the 1e-16 state differences into up to ~1e-9 relative differences
+  of the small momentum residuals.
+- Shipped case (8x64 cells x 32^2 velocities, the reference's own velocity
+  table) with workgroups of 64, 256 and 1024 threads: two steps
+  from the freestream start agree with the oracle as above, and with the
+  reference's release run as printed (dtPhysical and res(1:4) of
+  docs/release_checks/low_explicit/stdout.log, 7 significant digits).
+- Full size (64x512 cells x 64^2 velocities): two steps keep the total mass
+  to 1e-13 relative (x periodic, zero net mass flux through the diffuse
+  walls), keep the state finite and the cell densities positive, and give dt equal to
+  the CFL formula of the uniform start.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ugks2d_high_order_amd import config, velocity
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _nrel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _compare_steps(g, o, n):
+    for it in range(n):
+        rg, dtg = g.explicit_step()
+        st, ro, dto = o.explicit_step()
+        assert st == 0
+        e = [_nrel(x, y) for x, y in zip(g.state(), o.state())]
+        print("explicit step", it + 1, "W/h/b", e, "res", rg, ro, flush=True)
+        assert max(e) < 1e-12, (it, e)
+        assert abs(dtg - dto) <= 1e-15 * dto
+        assert np.all(np.abs(rg - ro) <= 1e-8 * np.abs(ro) + 1e-18), (rg, ro)
+    return rg, dtg
+
+
+def test_explicit_reduced_freestream_and_nonuniform(oracle_mod):
+    import ugks2d_high_order_amd as U
+    case = config.small_case()
+    tab = velocity.table(case.table, case.table_scale)
+    g = U.UGKS(case, *tab)
+    o = oracle_mod.OracleSolver(case, *tab, threads=8)
+    g.explicit_cold_start()
+    o.explicit_cold_start()
+    assert max(_nrel(x, y) for x, y in zip(g.state(), o.state())) < 1e-13
+    _compare_steps(g, o, 3)
+    o2 = oracle_mod.OracleSolver(case, *tab, threads=8)
+    o2.cold_start(1.0, 0.0, case.t_init)
+    for _ in range(3):
+        st, acc, _ = o2.iteration()
+        assert st == 0 and acc
+    g.set_state(*o2.state())
+    _compare_steps(g, o2, 3)
+
+
+@pytest.mark.parametrize("wg", [64, 256, 1024])
+def test_explicit_shipped_matches_oracle_and_release(oracle_mod, release, wg):
+    import ugks2d_high_order_amd as U
+    ex = json.load(open(os.path.join(HERE, "golden", "couette_explicit_release.json")))
+    case = config.shipped_case()
+    tab = (release["u"], release["v"], release["w"])
+    g = U.UGKS(case, *tab)
+    g.set_tuning(U.UGKS.TUNE_EXPLICIT_WG, wg)
+    o = oracle_mod.OracleSolver(case, *tab, threads=8)
+    g.explicit_cold_start()
+    o.explicit_cold_start()
+    rg, dtg = _compare_steps(g, o, ex["iteration"])
+    assert abs(dtg / ex["dt"] - 1) < 6e-7
+    assert np.max(np.abs(rg / np.asarray(ex["res"]) - 1)) < 6e-7, (rg, ex["res"])
+
+
+def test_explicit_full_size_conserves_mass():
+    import ugks2d_high_order_amd as U
+    case = config.couette_large()
+    g = U.UGKS(case)
+    g.explicit_cold_start()
+    W0, _, _ = g.state()
+    # uniform rest gas: dt = CFL*A/(dy*max(umax, c) + dx*max(vmax, c))
+    c = np.sqrt(0.5 * case.gamma * case.t_init)
+    umax, vmax = float(np.max(g.u)), float(np.max(g.v))
+    dt_ref = case.cfl * case.dx * case.dy / (case.dy * max(umax, c) + case.dx * max(vmax, c))
+    for it in range(2):
+        _, dt = g.explicit_step()
+        assert abs(dt / dt_ref - 1) < 1e-14
+    W, h, b = g.state()
+    print("full size mass drift", W[:, 0].sum() / W0[:, 0].sum() - 1, flush=True)
+    assert abs(W[:, 0].sum() / W0[:, 0].sum() - 1) < 1e-13
+    assert np.all(np.isfinite(h)) and np.all(np.isfinite(b))
+    assert W[:, 0].min() > 0

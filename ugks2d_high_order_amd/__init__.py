@@ -23,6 +23,7 @@ EXPORTS = [
     "ugks_evolution", "ugks_norms", "ugks_implicit_step", "ugks_run", "ugks_observables",
     "ugks_set_state_device", "ugks_get_state_device", "ugks_sync", "ugks_debug_fetch",
     "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning", "ugks_cn_step", "ugks_cn_ledger",
+    "ugks_explicit_step", "ugks_explicit_cold_start",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -73,6 +74,8 @@ def load_library(path: str = LIB_PATH):
     L.ugks_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.ugks_cn_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
     L.ugks_cn_ledger.argtypes = [C.c_void_p, _D]
+    L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
+    L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.ugks_observables.argtypes = [C.c_void_p, _D]
     L.ugks_sync.argtypes = [C.c_void_p]
     L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
@@ -198,6 +201,19 @@ class UGKS:
         _check(load_library().ugks_cn_ledger(self.handle, _ptr(o)))
         return o[:4], o[4:]
 
+    def explicit_cold_start(self, rho: float = 1.0, ux: float = 0.0, t_native: float | None = None):
+        """Freestream start of the low-order Explicit entry (InitFlow_Freestream)."""
+        t = self.case.t_init if t_native is None else t_native
+        _check(load_library().ugks_explicit_cold_start(self.handle, rho, ux, t))
+
+    def explicit_step(self):
+        """One step of the low-order Explicit entry; returns (res4, dt), the
+        residual res(1:4) Write_residuals prints and the global time step."""
+        r = np.zeros(4)
+        dt = np.zeros(1)
+        _check(load_library().ugks_explicit_step(self.handle, _ptr(r), _ptr(dt)))
+        return r, float(dt[0])
+
     def run(self, n: int) -> int:
         done = C.c_int(0)
         _check(load_library().ugks_run(self.handle, int(n), C.byref(done)))
@@ -228,6 +244,7 @@ class UGKS:
         return out
 
     TUNE_VEL_WG = 1
+    TUNE_EXPLICIT_WG = 2
 
     def set_tuning(self, key: int, value: int):
         _check(load_library().ugks_set_tuning(self.handle, int(key), int(value)))

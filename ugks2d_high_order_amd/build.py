@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libugks_hip.so")
-SOURCES = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_api.hip"]
+SOURCES = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_explicit.hip", "ugks_api.hip"]
 HEADERS = ["ugks_device.h", "ugks_recon.h", "ugks_state.h", "ugks_prof.h"]
 FORTRAN = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN, "couette_driver")

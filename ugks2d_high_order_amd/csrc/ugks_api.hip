@@ -25,6 +25,8 @@ extern "C" hipError_t ugks_launch_wall_reflection(const Dev* D, hipStream_t s, P
 extern "C" hipError_t ugks_launch_cn(const Dev* D, int build_old, double umax, double vmax, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
 extern "C" hipError_t ugks_launch_observables(const Dev* D, double* out, hipStream_t s);
+extern "C" hipError_t ugks_launch_lo_step(const Dev* D, int kpt, double umax, double vmax, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_lo_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
 
 namespace {
 
@@ -54,6 +56,7 @@ struct Instance {
     Prof prof;
     bool cn_alloc = false;        // S2CN2 target buffers allocated
     bool cn_targets_fresh = false; // tg holds the volume targets of the current state
+    int lo_wg = 256;              // workgroup size of the low-order Explicit kernels
 };
 
 // Polynomial coefficients (degrees 0..n-1) whose averages over the unit cells
@@ -428,6 +431,55 @@ extern "C" int ugks_cn_ledger(void* handle, double* out8) {
     return E_OK;
 }
 
+// the low-order Explicit kernels keep Gaussian tables per axis (Nu, Nv <= 64,
+// powers of two)
+static int lo_check(const Instance* I) {
+    const int nu = I->D.nu, nv = I->D.nv;
+    if (nu > 64 || nv > 64 || (nu & (nu - 1)) || (nv & (nv - 1)))
+        return fail(E_ARG, "the Explicit entry needs Nu, Nv powers of two up to 64");
+    if (I->lo_wg > I->D.nvel || nu + nv > I->lo_wg)
+        return fail(E_ARG, "Explicit workgroup size must be at least Nu+Nv and at most Nu*Nv");
+    return E_OK;
+}
+
+// Freestream cold start of the Explicit entry (InitFlow_Freestream): W of the
+// primitive state (rho, ux, 0, 1/t_native) and its Maxwellian projected on W.
+extern "C" int ugks_explicit_cold_start(void* handle, double rho, double ux, double t_native) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    if (!(rho > 0.0) || !(t_native > 0.0)) return fail(E_ARG, "cold start needs rho > 0 and T > 0");
+    int rc = lo_check(I);
+    if (rc) return rc;
+    HIPCHK(ugks_launch_lo_cold_start(&I->D, rho, ux, t_native, I->stream));
+    I->operator_fresh = false;
+    rc = check_status(I, nullptr);
+    return rc ? rc : reset_cn(I);
+}
+
+// One step of the low-order Explicit entry (TimeIntegration_method =
+// 'Explicit': TimeStep, EvolutionNative, ApplyExplicitUpdate,
+// src/mod_Solve.f90:84-165, src/mod_TimeIntegrator.f90:16-134).
+extern "C" int ugks_explicit_step(void* handle, double* res4, double* dt) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    int rc = lo_check(I);
+    if (rc) return rc;
+    Dev& D = I->D;
+    HIPCHK(ugks_launch_lo_step(&D, I->lo_wg, I->umax, I->vmax, I->stream, &I->prof));
+    double sc[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(sc, D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
+    if ((rc = check_status(I, nullptr))) return rc;
+    if (res4)
+        for (int m = 0; m < 4; ++m) res4[m] = sc[SC_LO_RES + m];
+    if (dt) *dt = sc[SC_DT];
+    std::swap(D.W, D.Wn);
+    std::swap(D.h, D.hn);
+    std::swap(D.b, D.bn);
+    I->operator_fresh = false;
+    I->cn_targets_fresh = false;
+    return E_OK;
+}
+
 extern "C" int ugks_run(void* handle, int n, int* done) {
     if (!handle || !done) return fail(E_ARG, "null argument");
     *done = 0;
@@ -496,6 +548,13 @@ extern "C" int ugks_set_tuning(void* handle, int key, int value) {
         if (value != 64 && value != 128 && value != 256) return fail(E_ARG, "velocity workgroup must be 64, 128 or 256");
         I->D.wg_vel = value;
         return E_OK;
+    case 2:
+        if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
+            return fail(E_ARG, "Explicit workgroup size must be 64, 128, 256, 512 or 1024");
+        if (value > I->D.nvel || I->D.nu + I->D.nv > value)
+            return fail(E_ARG, "Explicit workgroup size must be at least Nu+Nv and at most Nu*Nv");
+        I->lo_wg = value;
+        return E_OK;
     default: return fail(E_ARG, "unknown tuning key");
     }
 }
@@ -505,6 +564,7 @@ extern "C" int ugks_get_tuning(void* handle, int key, int* value) {
     if (!I || !value) return fail(E_ARG, "null argument");
     switch (key) {
     case 1: *value = I->D.wg_vel; return E_OK;
+    case 2: *value = I->lo_wg; return E_OK;
     default: return fail(E_ARG, "unknown tuning key");
     }
 }

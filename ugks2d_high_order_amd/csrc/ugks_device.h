@@ -82,6 +82,15 @@ __device__ inline double row16_max(double x) {
     x = fmax(x, dpp_f64<DPP_MIRROR>(x));
     return x;
 }
+// A workgroup-uniform double moved to scalar registers (the value is the same
+// in every lane).  Velocity loops then read per-face constants as scalar
+// operands instead of holding copies in vector registers.
+__device__ inline double uni(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
 // value of the partner lane (lane xor 1)
 __device__ inline double lane_xor1(double x) { return dpp_f64<DPP_XOR1>(x); }
 
@@ -139,6 +148,12 @@ __device__ inline void block_max(double* v, double* lds) {
 }
 
 #endif
+
+template <int N>
+__device__ inline void uni(double* x) {
+#pragma unroll
+    for (int a = 0; a < N; ++a) x[a] = uni(x[a]);
+}
 
 // moments (rho, rho U, rho V, rho E) contribution of one velocity
 __device__ inline void mom_add(double m[4], double w, double u, double v, double h, double b) {
@@ -209,6 +224,53 @@ __device__ inline bool solve4_gj(const double* m, const double* rhs, double* x) 
             for (int c = 0; c < 5; ++c) a[r][c] -= f * a[col][c];
         }
     }
+    for (int r = 0; r < 4; ++r) x[r] = a[r][4];
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+// The same Gauss-Jordan elimination (same pivot rule and floor) written so
+// that every array index is a compile-time constant: the row interchange is a
+// select over the candidate rows, and the matrix stays in registers instead
+// of private (scratch) memory.
+__device__ inline bool solve4_gj_reg(const double* m, const double* rhs, double* x) {
+    double a[4][5], scale = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { a[r][c] = m[c * 4 + r]; scale = fmax(scale, fabs(a[r][c])); }
+        a[r][4] = rhs[r];
+    }
+    scale = fmax(scale, UGKS_DBL_MIN);
+#pragma unroll
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        double best = fabs(a[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < 4; ++r)
+            if (fabs(a[r][col]) > best) { best = fabs(a[r][col]); piv = r; }
+        if (!(best > 1000.0 * UGKS_DBL_EPS * scale)) return false;
+#pragma unroll
+        for (int r = col + 1; r < 4; ++r) {
+            const bool sw = piv == r;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double t = a[col][c];
+                a[col][c] = sw ? a[r][c] : t;
+                a[r][c] = sw ? t : a[r][c];
+            }
+        }
+        const double d = a[col][col];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) a[col][c] /= d;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+#pragma unroll
     for (int r = 0; r < 4; ++r) x[r] = a[r][4];
     return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
 }

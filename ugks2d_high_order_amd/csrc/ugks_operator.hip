@@ -1320,6 +1320,11 @@ extern "C" hipError_t ugks_launch_cn(const Dev* Dp, int build_old, double umax, 
     }
 }
 
+extern "C" hipError_t ugks_launch_timestep(const Dev* Dp, double umax, double vmax, hipStream_t s, Prof* P) {
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, *Dp, umax, vmax));
+    return hipGetLastError();
+}
+
 extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vmax, double residual_tol,
                                            hipStream_t s, Prof* P) {
     const Dev& D = *Dp;

@@ -22,6 +22,7 @@ enum ScalarSlot {
     SC_CN_TOTAL0 = 64,  // 4 conserved totals at the start of the S2CN2 run
     SC_CN_BAL = 68,     // 4 cumulative wall balance (top - bottom, per unit length / dy)
     SC_CN_DIAG = 72,    // 16 diagnostics of the last S2CN2 step
+    SC_LO_RES = 88,     // residual res(1:4) of the last low-order Explicit step
     SC_COUNT = 96
 };
 

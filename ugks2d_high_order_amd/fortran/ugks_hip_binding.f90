@@ -88,6 +88,26 @@ module ugks_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: handle
     end function ugks_sync
+
+    integer(c_int) function ugks_cn_step(handle, accepted, diag16) bind(C, name="ugks_cn_step")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), intent(out) :: accepted
+      real(c_double), intent(out) :: diag16(16)
+    end function ugks_cn_step
+
+    integer(c_int) function ugks_explicit_step(handle, res4, dt) bind(C, name="ugks_explicit_step")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: res4(4), dt
+    end function ugks_explicit_step
+
+    integer(c_int) function ugks_explicit_cold_start(handle, rho, ux, t_native) &
+        bind(C, name="ugks_explicit_cold_start")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), value :: rho, ux, t_native
+    end function ugks_explicit_cold_start
   end interface
 
 contains
