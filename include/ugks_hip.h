@@ -157,9 +157,46 @@ int ugks_profile_read(void* handle, int idx, const char** name, double* total_ms
  *   1  workgroup size of the velocity-parallel operator kernels (face, wall,
  *      cell, cold start): 64, 128 or 256;
  *   2  workgroup size of the Explicit-entry face, wall and cell kernels:
- *      64, 128, 256 (default), 512 or 1024, at least Nu+Nv and at most Nvel. */
+ *      64, 128, 256 (default), 512 or 1024, at least Nu+Nv and at most Nvel;
+ *   3  velocity tile of the LU-SGS sweep workgroups: 8, 16 (default) or 32. */
 int ugks_set_tuning(void* handle, int key, int value);
 int ugks_get_tuning(void* handle, int key, int* value);
+
+/* ------------------------------------------------------------------------
+ * Batched instances (no counterpart in the reference, which runs one Couette
+ * case per MPI job).  n instances that share nx, the velocity table and the
+ * solver switches, with their own ny, walls, gas parameters and state, advance
+ * together: each stage of a HighOrderImplicit iteration is one launch over all
+ * instances, and one iteration of the whole batch is one replay of a captured
+ * hipGraph ending in a device-side acceptance (an instance whose update was
+ * rejected keeps its accepted state and is reported through its status word).
+ * Every instance computes exactly what ugks_implicit_step computes for it
+ * with the same launch shapes.
+ * ------------------------------------------------------------------------ */
+/* params: n parameter blocks; u, v, w: the shared velocity table. */
+int ugks_batch_create(int n, const ugks_params_t* params, const double* u, const double* v, const double* w,
+                      void** batch);
+void ugks_batch_destroy(void* batch);
+/* Handle of instance i, for the single-instance entry points (state access,
+ * norms, observables).  Valid until ugks_batch_destroy; do not destroy it. */
+int ugks_batch_instance(void* batch, int i, void** handle);
+/* Cold start of every instance (rest gas at rho[i], ux[i], t_native[i]). */
+int ugks_batch_cold_start(void* batch, const double* rho, const double* ux, const double* t_native);
+/* `iterations` graph replays.  accepted (n, may be NULL): accepted iterations
+ * per instance in this call; status (n, may be NULL): status bits of each
+ * instance's last iteration (0 = accepted). */
+int ugks_batch_run(void* batch, int iterations, int* accepted, unsigned* status);
+/* The same iterations launched stage by stage without the graph; kernel
+ * times are recorded in instance 0's profile (ugks_profile on its handle). */
+int ugks_batch_run_eager(void* batch, int iterations, int* accepted, unsigned* status);
+/* Launch shapes of every instance: velocity workgroup of the face, wall and
+ * cell kernels (64, 128, 256) and velocity tile of the LU-SGS sweeps (8, 16, 32). */
+int ugks_batch_set_shape(void* batch, int wg_vel, int tile);
+/* Autotuner: times `reps` replays of each shape, restores the instances'
+ * state before each and at the end, and keeps the fastest.  out2: the chosen
+ * (wg_vel, tile); ms9 (may be NULL): ms per iteration of each candidate,
+ * row-major over wg_vel {64,128,256} x tile {8,16,32}. */
+int ugks_batch_autotune(void* batch, int reps, int* out2, double* ms9);
 
 #ifdef __cplusplus
 }

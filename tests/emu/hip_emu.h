@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -90,6 +91,8 @@ inline hipError_t hipMalloc(void** p, size_t n) {
     std::memset(*p, 0xff, n);
     return hipSuccess;
 }
+template <typename T>
+inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMemset(void* p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
@@ -108,8 +111,34 @@ inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { retu
 template <typename T>
 inline hipError_t hipMemcpyToSymbol(T& sym, const void* src, size_t n) { std::memcpy((void*)&sym, src, n); return hipSuccess; }
 
+// stream capture: launches made while capturing are recorded and replayed
+// in order by hipGraphLaunch
+struct EmuGraph { std::vector<std::function<void()>> ops; };
+typedef EmuGraph* hipGraph_t;
+typedef EmuGraph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline thread_local EmuGraph* emu_capture = nullptr;
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { emu_capture = new EmuGraph(); return hipSuccess; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = emu_capture; emu_capture = nullptr; return hipSuccess; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, size_t) { *e = g; return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) { for (auto& op : e->ops) op(); return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+
 template <typename F, typename... A>
-inline void emu_launch(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args) {
+inline void emu_launch_now(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args);
+
+template <typename F, typename... A>
+inline void emu_launch(F f, dim3 g, dim3 b, size_t shmem, hipStream_t st, A... args) {
+    if (emu_capture) {
+        emu_capture->ops.push_back([=]() { emu_launch_now(f, g, b, shmem, st, args...); });
+        return;
+    }
+    emu_launch_now(f, g, b, shmem, st, args...);
+}
+
+template <typename F, typename... A>
+inline void emu_launch_now(F f, dim3 g, dim3 b, size_t shmem, hipStream_t, A... args) {
     std::vector<char> dyn(shmem + 64, (char)0xff);
     const int nt = (int)(b.x * b.y * b.z);
     const long nb = (long)g.x * g.y * g.z;

@@ -24,6 +24,8 @@ EXPORTS = [
     "ugks_set_state_device", "ugks_get_state_device", "ugks_sync", "ugks_debug_fetch",
     "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning", "ugks_cn_step", "ugks_cn_ledger",
     "ugks_explicit_step", "ugks_explicit_cold_start",
+    "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
+    "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -76,6 +78,15 @@ def load_library(path: str = LIB_PATH):
     L.ugks_cn_ledger.argtypes = [C.c_void_p, _D]
     L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
     L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.ugks_batch_create.argtypes = [C.c_int, C.POINTER(Params), _D, _D, _D, C.POINTER(C.c_void_p)]
+    L.ugks_batch_destroy.argtypes = [C.c_void_p]
+    L.ugks_batch_destroy.restype = None
+    L.ugks_batch_instance.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+    L.ugks_batch_cold_start.argtypes = [C.c_void_p, _D, _D, _D]
+    L.ugks_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint)]
+    L.ugks_batch_run_eager.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint)]
+    L.ugks_batch_set_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ugks_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.ugks_observables.argtypes = [C.c_void_p, _D]
     L.ugks_sync.argtypes = [C.c_void_p]
     L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
@@ -245,6 +256,7 @@ class UGKS:
 
     TUNE_VEL_WG = 1
     TUNE_EXPLICIT_WG = 2
+    TUNE_SWEEP_TILE = 3
 
     def set_tuning(self, key: int, value: int):
         _check(load_library().ugks_set_tuning(self.handle, int(key), int(value)))
@@ -256,6 +268,89 @@ class UGKS:
 
     def sync(self):
         _check(load_library().ugks_sync(self.handle))
+
+
+class _InstanceView(UGKS):
+    """Instance i of a Batch: the single-instance methods on its handle,
+    which the batch owns (closing the view does not destroy it)."""
+
+    def __init__(self, batch, i, handle):
+        self.case = batch.cases[i]
+        self.u, self.v, self.w = batch.u, batch.v, batch.w
+        self.nvel = self.u.size
+        self.ncell = self.case.nx * self.case.ny
+        self.handle = handle
+        self._batch = batch
+
+    def close(self):
+        self.handle = None
+
+
+class Batch:
+    """Many Couette instances advanced together (ugks_batch_*): one launch per
+    stage over all instances, one captured hipGraph replay per iteration.
+    Instances share nx, the velocity table and the solver switches; ny, walls,
+    Kn and temperatures may differ."""
+
+    def __init__(self, cases, u=None, v=None, w=None):
+        self.cases = list(cases)
+        c0 = self.cases[0]
+        if u is None:
+            u, v, w = velocity.table(c0.table, c0.table_scale)
+        self.u = np.ascontiguousarray(u, dtype=np.float64)
+        self.v = np.ascontiguousarray(v, dtype=np.float64)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        L = load_library()
+        n = len(self.cases)
+        self._p = (Params * n)(*[make_params(c) for c in self.cases])
+        h = C.c_void_p()
+        _check(L.ugks_batch_create(n, self._p, _ptr(self.u), _ptr(self.v), _ptr(self.w), C.byref(h)))
+        self.handle = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().ugks_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def instance(self, i: int) -> UGKS:
+        h = C.c_void_p()
+        _check(load_library().ugks_batch_instance(self.handle, int(i), C.byref(h)))
+        return _InstanceView(self, i, h)
+
+    def cold_start(self, rho=None, ux=None, t_native=None):
+        n = self.n
+        rho = np.ones(n) if rho is None else np.asarray(rho, dtype=np.float64)
+        ux = np.zeros(n) if ux is None else np.asarray(ux, dtype=np.float64)
+        t = np.array([c.t_init for c in self.cases]) if t_native is None else np.asarray(t_native, dtype=np.float64)
+        _check(load_library().ugks_batch_cold_start(self.handle, _ptr(rho), _ptr(ux), _ptr(t)))
+
+    def run(self, iterations: int, graph: bool = True):
+        """Advance every instance; returns (accepted iterations, status words)."""
+        acc = (C.c_int * self.n)()
+        st = (C.c_uint * self.n)()
+        f = load_library().ugks_batch_run if graph else load_library().ugks_batch_run_eager
+        _check(f(self.handle, int(iterations), acc, st))
+        return np.array(acc[:]), np.array(st[:])
+
+    def set_shape(self, wg_vel: int, tile: int):
+        _check(load_library().ugks_batch_set_shape(self.handle, int(wg_vel), int(tile)))
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest (velocity workgroup, sweep tile); returns
+        ((wg_vel, tile), {(wg_vel, tile): ms per iteration})."""
+        out = (C.c_int * 2)()
+        ms = np.zeros(9)
+        _check(load_library().ugks_batch_autotune(self.handle, int(reps), out, _ptr(ms)))
+        table = {(wg, t): ms[3 * a + b] for a, wg in enumerate((64, 128, 256)) for b, t in enumerate((8, 16, 32))
+                 if ms[3 * a + b] > 0}
+        return (out[0], out[1]), table
 
 
 def solve_ugks(case, max_iter: int | None = None, streak_needed: int = 5, callback=None, solver: UGKS | None = None):

@@ -97,6 +97,30 @@ def small_case(**kw) -> CouetteCase:
     return dataclasses.replace(c, **kw)
 
 
+def batched_cases(n: int = 64, seed: int = 20261203, nx: int = 8, ny_choices=(32, 64, 96, 128), nu: int = 32,
+                  table: str = "half32", t_init: float = 2.0, **kw) -> list:
+    """Ragged batch of closed Couette instances drawn from a seed (NumPy
+    PCG64): ny uniform over ny_choices, Kn log-uniform in [0.01, 1]
+    (mu_ref = Kn/sqrt(pi)), wall Mach uniform in [0.5, 3] on the wall sound
+    speed, and wall/initial temperature ratio uniform in [0.8, 1.2] with the
+    initial gas at t_init.  Walls move at -U (bottom) and +U (top)."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ny = rng.choice(np.asarray(ny_choices), size=n)
+    kn = np.exp(rng.uniform(math.log(0.01), math.log(1.0), size=n))
+    ma = rng.uniform(0.5, 3.0, size=n)
+    tr = rng.uniform(0.8, 1.2, size=n)
+    out = []
+    for i in range(n):
+        tw = float(tr[i]) * t_init
+        u = wall_speed(float(ma[i]), tw)
+        c = CouetteCase(nx=nx, ny=int(ny[i]), nu=nu, nv=nu, table=table, omega=0.81,
+                        mu_ref=float(kn[i]) / math.sqrt(math.pi), wall_ux=(-u, u), wall_tn=(tw, tw), t_init=t_init,
+                        max_iter=300)
+        out.append(dataclasses.replace(c, **kw))
+    return out
+
+
 _NAMELIST_KEYS = {
     "nx_all": ("nx", int), "ny_all": ("ny", int), "nu": ("nu", int), "nv": ("nv", int),
     "ck": ("ck", int), "pr": ("pr", float), "omega": ("omega", float), "couettemuref": ("mu_ref", float),

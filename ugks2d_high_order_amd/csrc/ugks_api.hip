@@ -18,19 +18,21 @@
 #include "ugks_prof.h"
 
 extern "C" hipError_t ugks_set_consts(const Consts* k);
-extern "C" hipError_t ugks_launch_operator(const Dev* D, double umax, double vmax, double residual_tol, hipStream_t s,
-                                           Prof* P);
-extern "C" hipError_t ugks_launch_implicit(const Dev* D, int rescale, hipStream_t s, Prof* P);
-extern "C" hipError_t ugks_launch_wall_reflection(const Dev* D, hipStream_t s, Prof* P);
-extern "C" hipError_t ugks_launch_cn(const Dev* D, int build_old, double umax, double vmax, hipStream_t s, Prof* P);
-extern "C" hipError_t ugks_launch_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
-extern "C" hipError_t ugks_launch_observables(const Dev* D, double* out, hipStream_t s);
-extern "C" hipError_t ugks_launch_lo_step(const Dev* D, int kpt, double umax, double vmax, hipStream_t s, Prof* P);
-extern "C" hipError_t ugks_launch_lo_cold_start(const Dev* D, double rho, double ux, double tn, hipStream_t s);
+extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_implicit(const LaunchSet* L, int rescale, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_wall_reflection(const LaunchSet* L, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_cn(const LaunchSet* L, int build_old, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s);
+extern "C" hipError_t ugks_launch_observables(const LaunchSet* L, double* out, hipStream_t s);
+extern "C" hipError_t ugks_launch_lo_step(const LaunchSet* L, int wg, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_lo_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s);
 
 namespace {
 
 thread_local std::string g_err;
+
+// smallest velocity tile of the LU-SGS sweeps (the per-tile partial buffers are sized for it)
+constexpr int MIN_TILE = 8;
 
 enum { E_OK = 0, E_NONFINITE = 1, E_GATE = 2, E_INADMISSIBLE = 3, E_POLICY = 4, E_NEWTON = 5, E_DEVICE = 6, E_ARG = 7 };
 
@@ -47,9 +49,9 @@ int fail(int code, const std::string& msg) {
 
 struct Instance {
     ugks_params_t p;
-    Dev D;
+    Dev D;                        // host copy of the descriptor
+    Dev* dD = nullptr;            // device copy the kernels read
     hipStream_t stream = nullptr;
-    double umax = 0, vmax = 0;
     bool operator_fresh = false;
     std::vector<void*> allocs;
     double* obs = nullptr;
@@ -162,8 +164,19 @@ int check_status(Instance* I, unsigned* out_status) {
     return E_OK;
 }
 
+// launch target of one instance
+LaunchSet one(const Instance* I) { return LaunchSet{&I->D, I->dD, 1}; }
+
+// publish the host descriptor to the device copy (after a pointer swap or a
+// new buffer); stream-ordered before the next launch
+int upload(Instance* I) {
+    HIPCHK(hipMemcpyAsync(I->dD, &I->D, sizeof(Dev), hipMemcpyHostToDevice, I->stream));
+    return E_OK;
+}
+
 int run_operator(Instance* I) {
-    HIPCHK(ugks_launch_operator(&I->D, I->umax, I->vmax, I->p.residual_tol, I->stream, &I->prof));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_operator(&L, I->stream, &I->prof));
     int rc = check_status(I, nullptr);
     if (rc) return rc;
     I->operator_fresh = true;
@@ -202,8 +215,9 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     D.tile = 16;
     D.wg_vel = 128;
     D.ntiles = nvel / D.tile;
-    I->umax = -1e300; I->vmax = -1e300;
-    for (int k = 0; k < nvel; ++k) { I->umax = std::max(I->umax, u[k]); I->vmax = std::max(I->vmax, v[k]); }
+    D.umax = -1e300; D.vmax = -1e300;
+    for (int k = 0; k < nvel; ++k) { D.umax = std::max(D.umax, u[k]); D.vmax = std::max(D.vmax, v[k]); }
+    D.residual_tol = p->residual_tol;
     const size_t nc = D.ncell, cv = nc * nvel, nx = D.nx, ny = D.ny;
     auto bail = [&](int code) { ugks_destroy(I); return code; };
     if (hipStreamCreate(&I->stream) != hipSuccess) { delete I; return fail(E_DEVICE, "hipStreamCreate failed"); }
@@ -221,15 +235,17 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
         (rc = dmalloc(I, &D.ws, nc * 4 * 24)) || (rc = dmalloc(I, &D.wv, nc * 4 * 9)) ||
         (rc = dmalloc(I, &D.cellpart, nc * 16)) || (rc = dmalloc(I, &D.gx, nc)) ||
         (rc = dmalloc(I, &D.gy, (ny + 1) * nx)) || (rc = dmalloc(I, &D.tau_t, nc)) ||
-        (rc = dmalloc(I, &D.wallpart, (size_t)D.ntiles * 2 * nx)) ||
-        (rc = dmalloc(I, &D.mompart, (size_t)D.ntiles * nc * 4)) || (rc = dmalloc(I, &D.scal, SC_COUNT)) ||
-        (rc = dmalloc(I, &D.status, 1)) || (rc = dmalloc(I, &I->obs, nc * 10)))
+        (rc = dmalloc(I, &D.wallpart, (size_t)(nvel / MIN_TILE) * 2 * nx)) ||
+        (rc = dmalloc(I, &D.mompart, (size_t)(nvel / MIN_TILE) * nc * 4)) || (rc = dmalloc(I, &D.scal, SC_COUNT)) ||
+        (rc = dmalloc(I, &D.status, 1)) || (rc = dmalloc(I, &I->obs, nc * 10)) || (rc = dmalloc(I, &I->dD, 1)))
         return bail(rc);
     if (hipMemcpy(du, u, sizeof(double) * nvel, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dv, v, sizeof(double) * nvel, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dw, w, sizeof(double) * nvel, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(E_DEVICE, "velocity table upload failed"));
-    if (ugks_launch_wall_reflection(&D, I->stream, &I->prof) != hipSuccess) return bail(fail(E_DEVICE, "launch failed"));
+    if ((rc = upload(I))) return bail(rc);
+    LaunchSet L = one(I);
+    if (ugks_launch_wall_reflection(&L, I->stream, &I->prof) != hipSuccess) return bail(fail(E_DEVICE, "launch failed"));
     if ((rc = check_status(I, nullptr))) return bail(rc);
     *handle = I;
     return E_OK;
@@ -297,7 +313,8 @@ extern "C" int ugks_cold_start(void* handle, double rho, double ux, double t_nat
     Instance* I = (Instance*)handle;
     if (!I) return fail(E_ARG, "null handle");
     if (!(rho > 0.0) || !(t_native > 0.0)) return fail(E_ARG, "cold start needs rho > 0 and T > 0");
-    HIPCHK(ugks_launch_cold_start(&I->D, rho, ux, t_native, I->stream));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_cold_start(&L, rho, ux, t_native, I->stream));
     I->operator_fresh = false;
     int rc = check_status(I, nullptr);
     return rc ? rc : reset_cn(I);
@@ -348,7 +365,8 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
         return fail(E_POLICY, "closed-Couette policy requires nCFL = 1000 (CheckR5Policy)");
     int rc = E_OK;
     if (!I->operator_fresh && (rc = run_operator(I))) return rc;
-    HIPCHK(ugks_launch_implicit(&I->D, p.mass_rescale ? 1 : 0, I->stream, &I->prof));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_implicit(&L, p.mass_rescale ? 1 : 0, I->stream, &I->prof));
     double sc[SC_COUNT];
     unsigned st = 0;
     HIPCHK(hipMemcpyAsync(sc, I->D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
@@ -373,6 +391,7 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
     std::swap(I->D.W, I->D.Wn);
     std::swap(I->D.h, I->D.hn);
     std::swap(I->D.b, I->D.bn);
+    if ((rc = upload(I))) return rc;
     I->operator_fresh = false;
     I->cn_targets_fresh = false;
     *accepted = 1;
@@ -393,10 +412,12 @@ extern "C" int ugks_cn_step(void* handle, int* accepted, double* diag16) {
         for (int q = 0; q < 6; ++q)
             if ((rc = dmalloc(I, &D.tg[q], cv)) || (rc = dmalloc(I, &D.tgn[q], cv))) return rc;
         if ((rc = dmalloc(I, &D.cnpart, (size_t)D.ncell * 8))) return rc;
+        if ((rc = upload(I))) return rc;
         I->cn_alloc = true;
         I->cn_targets_fresh = false;
     }
-    HIPCHK(ugks_launch_cn(&D, I->cn_targets_fresh ? 0 : 1, I->umax, I->vmax, I->stream, &I->prof));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_cn(&L, I->cn_targets_fresh ? 0 : 1, I->stream, &I->prof));
     double sc[SC_COUNT];
     unsigned st = 0;
     HIPCHK(hipMemcpyAsync(sc, D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
@@ -415,6 +436,7 @@ extern "C" int ugks_cn_step(void* handle, int* accepted, double* diag16) {
     std::swap(D.h, D.hn);
     std::swap(D.b, D.bn);
     for (int q = 0; q < 6; ++q) std::swap(D.tg[q], D.tgn[q]);
+    if ((rc = upload(I))) return rc;
     *accepted = 1;
     return E_OK;
 }
@@ -450,7 +472,8 @@ extern "C" int ugks_explicit_cold_start(void* handle, double rho, double ux, dou
     if (!(rho > 0.0) || !(t_native > 0.0)) return fail(E_ARG, "cold start needs rho > 0 and T > 0");
     int rc = lo_check(I);
     if (rc) return rc;
-    HIPCHK(ugks_launch_lo_cold_start(&I->D, rho, ux, t_native, I->stream));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_lo_cold_start(&L, rho, ux, t_native, I->stream));
     I->operator_fresh = false;
     rc = check_status(I, nullptr);
     return rc ? rc : reset_cn(I);
@@ -465,7 +488,8 @@ extern "C" int ugks_explicit_step(void* handle, double* res4, double* dt) {
     int rc = lo_check(I);
     if (rc) return rc;
     Dev& D = I->D;
-    HIPCHK(ugks_launch_lo_step(&D, I->lo_wg, I->umax, I->vmax, I->stream, &I->prof));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_lo_step(&L, I->lo_wg, I->stream, &I->prof));
     double sc[SC_COUNT];
     HIPCHK(hipMemcpyAsync(sc, D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
     if ((rc = check_status(I, nullptr))) return rc;
@@ -475,6 +499,7 @@ extern "C" int ugks_explicit_step(void* handle, double* res4, double* dt) {
     std::swap(D.W, D.Wn);
     std::swap(D.h, D.hn);
     std::swap(D.b, D.bn);
+    if ((rc = upload(I))) return rc;
     I->operator_fresh = false;
     I->cn_targets_fresh = false;
     return E_OK;
@@ -547,7 +572,13 @@ extern "C" int ugks_set_tuning(void* handle, int key, int value) {
     case 1:
         if (value != 64 && value != 128 && value != 256) return fail(E_ARG, "velocity workgroup must be 64, 128 or 256");
         I->D.wg_vel = value;
-        return E_OK;
+        return upload(I);
+    case 3:
+        if (value != 8 && value != 16 && value != 32) return fail(E_ARG, "sweep velocity tile must be 8, 16 or 32");
+        if (I->D.nvel % value) return fail(E_ARG, "sweep velocity tile must divide Nvel");
+        I->D.tile = value;
+        I->D.ntiles = I->D.nvel / value;
+        return upload(I);
     case 2:
         if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
             return fail(E_ARG, "Explicit workgroup size must be 64, 128, 256, 512 or 1024");
@@ -565,6 +596,7 @@ extern "C" int ugks_get_tuning(void* handle, int key, int* value) {
     switch (key) {
     case 1: *value = I->D.wg_vel; return E_OK;
     case 2: *value = I->lo_wg; return E_OK;
+    case 3: *value = I->D.tile; return E_OK;
     default: return fail(E_ARG, "unknown tuning key");
     }
 }
@@ -572,8 +604,328 @@ extern "C" int ugks_get_tuning(void* handle, int key, int* value) {
 extern "C" int ugks_observables(void* handle, double* fields10) {
     Instance* I = (Instance*)handle;
     if (!I || !fields10) return fail(E_ARG, "null argument");
-    HIPCHK(ugks_launch_observables(&I->D, I->obs, I->stream));
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_observables(&L, I->obs, I->stream));
     HIPCHK(hipMemcpyAsync(fields10, I->obs, sizeof(double) * 10 * I->D.ncell, hipMemcpyDeviceToHost, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
+// ===========================================================================
+// Batched instances (no counterpart in the reference)
+// ===========================================================================
+// Every instance keeps its own buffers (created through the single-instance
+// path); the batch adds a device array of their descriptors, so each stage of
+// an iteration is one launch over all instances (grid z = instance), and one
+// iteration of the whole batch is a captured hipGraph: operator, implicit
+// update and the device-side acceptance (pointer swap of accepted instances).
+// Who holds the current descriptors alternates: after a batch run the device
+// array does (pointer swaps happened there); the single-instance entry points
+// used by the batch (cold start, state access) first pull them back.
+extern "C" hipError_t ugks_launch_batch_accept(Dev* DS, int n, int* count, unsigned* status_out, hipStream_t s);
+extern "C" hipError_t ugks_launch_zero_ints(int* p, int n, hipStream_t s);
+
+namespace {
+
+struct Batch {
+    int n = 0;
+    int rescale = 0;
+    std::vector<Instance*> inst;
+    Dev size;                       // sizing descriptor (largest ny)
+    Dev* dDs = nullptr;             // device array of the n descriptors
+    int* dcount = nullptr;          // accepted iterations per instance
+    unsigned* dstat = nullptr;      // status word per instance after the last iteration
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool device_truth = false;      // dDs holds the current descriptors
+};
+
+void batch_drop_graph(Batch* B) {
+    if (B->exec) hipGraphExecDestroy(B->exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+}
+
+// descriptors: device array -> instances (host copies and their own device copies)
+int batch_pull(Batch* B) {
+    if (!B->device_truth) return E_OK;
+    std::vector<Dev> tmp(B->n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), B->dDs, sizeof(Dev) * B->n, hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (int i = 0; i < B->n; ++i) {
+        Instance* I = B->inst[i];
+        I->D = tmp[i];
+        int rc = upload(I);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(I->stream));
+        I->operator_fresh = false;
+        I->cn_targets_fresh = false;
+    }
+    B->device_truth = false;
+    return E_OK;
+}
+
+// descriptors: instances -> device array
+int batch_push(Batch* B) {
+    if (B->device_truth) return E_OK;
+    std::vector<Dev> tmp(B->n);
+    for (int i = 0; i < B->n; ++i) {
+        HIPCHK(hipStreamSynchronize(B->inst[i]->stream));
+        tmp[i] = B->inst[i]->D;
+    }
+    HIPCHK(hipMemcpyAsync(B->dDs, tmp.data(), sizeof(Dev) * B->n, hipMemcpyHostToDevice, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    B->device_truth = true;
+    return E_OK;
+}
+
+// one iteration of every instance, as stream work on B->stream
+int batch_iteration(Batch* B, Prof* P) {
+    LaunchSet L{&B->size, B->dDs, B->n};
+    HIPCHK(ugks_launch_operator(&L, B->stream, P));
+    HIPCHK(ugks_launch_implicit(&L, B->rescale, B->stream, P));
+    HIPCHK(ugks_launch_batch_accept(B->dDs, B->n, B->dcount, B->dstat, B->stream));
+    return E_OK;
+}
+
+int batch_capture(Batch* B) {
+    if (B->exec) return E_OK;
+    HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_iteration(B, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(B->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    B->graph = g;
+    HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    return E_OK;
+}
+
+// shapes the tuning keys give every instance
+int batch_set_shape(Batch* B, int wg_vel, int tile) {
+    int rc = batch_pull(B);
+    if (rc) return rc;
+    for (Instance* I : B->inst) {
+        if ((rc = ugks_set_tuning(I, 1, wg_vel)) || (rc = ugks_set_tuning(I, 3, tile))) return rc;
+        HIPCHK(hipStreamSynchronize(I->stream));
+    }
+    B->size.wg_vel = wg_vel;
+    B->size.tile = tile;
+    B->size.ntiles = B->size.nvel / tile;
+    batch_drop_graph(B);
+    return E_OK;
+}
+
+}  // namespace
+
+extern "C" int ugks_batch_create(int n, const ugks_params_t* params, const double* u, const double* v, const double* w,
+                                 void** batch) {
+    if (n < 1 || !params || !u || !v || !w || !batch) return fail(E_ARG, "invalid batch arguments");
+    *batch = nullptr;
+    const ugks_params_t& p0 = params[0];
+    for (int i = 1; i < n; ++i) {
+        const ugks_params_t& p = params[i];
+        if (p.nx != p0.nx || p.nu != p0.nu || p.nv != p0.nv || p.ck != p0.ck || p.macro_sweeps != p0.macro_sweeps ||
+            p.micro_sweeps != p0.micro_sweeps || p.mass_rescale != p0.mass_rescale)
+            return fail(E_ARG, "batched instances must share nx, Nu, Nv, ck, the sweep counts and the mass rescale");
+    }
+    if (p0.mass_rescale)
+        for (int i = 0; i < n; ++i)
+            if (!(params[i].ncfl == 1000.0)) return fail(E_POLICY, "closed-Couette policy requires nCFL = 1000");
+    Batch* B = new Batch();
+    B->n = n;
+    B->rescale = p0.mass_rescale ? 1 : 0;
+    auto bail = [&](int code) {
+        for (Instance* I : B->inst) ugks_destroy(I);
+        if (B->dDs) hipFree(B->dDs);
+        if (B->dcount) hipFree(B->dcount);
+        if (B->dstat) hipFree(B->dstat);
+        if (B->stream) hipStreamDestroy(B->stream);
+        delete B;
+        return code;
+    };
+    for (int i = 0; i < n; ++i) {
+        void* h = nullptr;
+        int rc = ugks_create(&params[i], u, v, w, &h);
+        if (rc) return bail(rc);
+        B->inst.push_back((Instance*)h);
+    }
+    B->size = B->inst[0]->D;
+    for (Instance* I : B->inst)
+        if (I->D.ny > B->size.ny) { B->size.ny = I->D.ny; B->size.ncell = I->D.ncell; }
+    if (hipStreamCreate(&B->stream) != hipSuccess || hipMalloc(&B->dDs, sizeof(Dev) * n) != hipSuccess ||
+        hipMalloc(&B->dcount, sizeof(int) * n) != hipSuccess || hipMalloc(&B->dstat, sizeof(unsigned) * n) != hipSuccess)
+        return bail(fail(E_DEVICE, "batch allocation failed"));
+    int rc = batch_push(B);
+    if (rc) return bail(rc);
+    *batch = B;
+    return E_OK;
+}
+
+extern "C" void ugks_batch_destroy(void* batch) {
+    Batch* B = (Batch*)batch;
+    if (!B) return;
+    if (B->stream) hipStreamSynchronize(B->stream);
+    batch_drop_graph(B);
+    for (Instance* I : B->inst) ugks_destroy(I);
+    if (B->dDs) hipFree(B->dDs);
+    if (B->dcount) hipFree(B->dcount);
+    if (B->dstat) hipFree(B->dstat);
+    if (B->stream) hipStreamDestroy(B->stream);
+    delete B;
+}
+
+extern "C" int ugks_batch_instance(void* batch, int i, void** handle) {
+    Batch* B = (Batch*)batch;
+    if (!B || !handle || i < 0 || i >= B->n) return fail(E_ARG, "invalid instance index");
+    int rc = batch_pull(B);
+    if (rc) return rc;
+    *handle = B->inst[i];
+    return E_OK;
+}
+
+extern "C" int ugks_batch_cold_start(void* batch, const double* rho, const double* ux, const double* t_native) {
+    Batch* B = (Batch*)batch;
+    if (!B || !rho || !ux || !t_native) return fail(E_ARG, "null argument");
+    int rc = batch_pull(B);
+    if (rc) return rc;
+    for (int i = 0; i < B->n; ++i)
+        if ((rc = ugks_cold_start(B->inst[i], rho[i], ux[i], t_native[i]))) return rc;
+    return E_OK;
+}
+
+extern "C" int ugks_batch_run(void* batch, int iterations, int* accepted, unsigned* status) {
+    Batch* B = (Batch*)batch;
+    if (!B || iterations < 0) return fail(E_ARG, "invalid batch run arguments");
+    int rc = batch_push(B);
+    if (rc) return rc;
+    HIPCHK(ugks_launch_zero_ints(B->dcount, B->n, B->stream));
+    if ((rc = batch_capture(B))) return rc;
+    for (int it = 0; it < iterations; ++it) HIPCHK(hipGraphLaunch(B->exec, B->stream));
+    if (accepted) HIPCHK(hipMemcpyAsync(accepted, B->dcount, sizeof(int) * B->n, hipMemcpyDeviceToHost, B->stream));
+    if (status) HIPCHK(hipMemcpyAsync(status, B->dstat, sizeof(unsigned) * B->n, hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return E_OK;
+}
+
+// Same iterations launched stage by stage without a graph (for comparison
+// and for timing each kernel through ugks_batch_profile).
+extern "C" int ugks_batch_run_eager(void* batch, int iterations, int* accepted, unsigned* status) {
+    Batch* B = (Batch*)batch;
+    if (!B || iterations < 0) return fail(E_ARG, "invalid batch run arguments");
+    int rc = batch_push(B);
+    if (rc) return rc;
+    HIPCHK(ugks_launch_zero_ints(B->dcount, B->n, B->stream));
+    Prof* P = &B->inst[0]->prof;
+    for (int it = 0; it < iterations; ++it)
+        if ((rc = batch_iteration(B, P))) return rc;
+    if (accepted) HIPCHK(hipMemcpyAsync(accepted, B->dcount, sizeof(int) * B->n, hipMemcpyDeviceToHost, B->stream));
+    if (status) HIPCHK(hipMemcpyAsync(status, B->dstat, sizeof(unsigned) * B->n, hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return E_OK;
+}
+
+extern "C" int ugks_batch_set_shape(void* batch, int wg_vel, int tile) {
+    Batch* B = (Batch*)batch;
+    if (!B) return fail(E_ARG, "null batch");
+    return batch_set_shape(B, wg_vel, tile);
+}
+
+// Autotuner (no counterpart in the reference): time `reps` graph replays of
+// every candidate shape (velocity workgroup of the face/wall/cell kernels x
+// velocity tile of the LU-SGS sweeps), restoring the instances' state and
+// scalars before each candidate and at the end, then keep the fastest.
+// out: [best wg_vel, best tile]; ms (may be NULL): 9 timings in ms per
+// iteration, row-major over wg_vel {64,128,256} x tile {8,16,32} (0 where a
+// candidate does not apply).
+extern "C" int ugks_batch_autotune(void* batch, int reps, int* out2, double* ms9) {
+    Batch* B = (Batch*)batch;
+    if (!B || !out2 || reps < 1) return fail(E_ARG, "invalid autotune arguments");
+    int rc = batch_pull(B);
+    if (rc) return rc;
+    // snapshot of every instance: descriptor, state and scalars
+    const int n = B->n;
+    std::vector<Dev> saved(n);
+    std::vector<void*> snap;
+    auto release = [&]() { for (void* q : snap) hipFree(q); };
+    for (int i = 0; i < n; ++i) {
+        Instance* I = B->inst[i];
+        saved[i] = I->D;
+        const size_t nc = I->D.ncell, cv = nc * I->D.nvel;
+        const size_t bytes = sizeof(double) * (4 * nc + 2 * cv + SC_COUNT) + sizeof(unsigned);
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) { release(); return fail(E_DEVICE, "autotune snapshot allocation failed"); }
+        snap.push_back(q);
+        char* c = (char*)q;
+        HIPCHK(hipMemcpyAsync(c, I->D.W, sizeof(double) * 4 * nc, hipMemcpyDeviceToDevice, B->stream));
+        c += sizeof(double) * 4 * nc;
+        HIPCHK(hipMemcpyAsync(c, I->D.h, sizeof(double) * cv, hipMemcpyDeviceToDevice, B->stream));
+        c += sizeof(double) * cv;
+        HIPCHK(hipMemcpyAsync(c, I->D.b, sizeof(double) * cv, hipMemcpyDeviceToDevice, B->stream));
+        c += sizeof(double) * cv;
+        HIPCHK(hipMemcpyAsync(c, I->D.scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToDevice, B->stream));
+        c += sizeof(double) * SC_COUNT;
+        HIPCHK(hipMemcpyAsync(c, I->D.status, sizeof(unsigned), hipMemcpyDeviceToDevice, B->stream));
+    }
+    HIPCHK(hipStreamSynchronize(B->stream));
+    auto restore = [&]() -> int {
+        int r = batch_pull(B);
+        if (r) return r;
+        for (int i = 0; i < n; ++i) {
+            Instance* I = B->inst[i];
+            const int wg = I->D.wg_vel, tile = I->D.tile, ntiles = I->D.ntiles;
+            I->D = saved[i];
+            I->D.wg_vel = wg; I->D.tile = tile; I->D.ntiles = ntiles;
+            const size_t nc = I->D.ncell, cv = nc * I->D.nvel;
+            const char* c = (const char*)snap[i];
+            HIPCHK(hipMemcpyAsync(I->D.W, c, sizeof(double) * 4 * nc, hipMemcpyDeviceToDevice, B->stream));
+            c += sizeof(double) * 4 * nc;
+            HIPCHK(hipMemcpyAsync(I->D.h, c, sizeof(double) * cv, hipMemcpyDeviceToDevice, B->stream));
+            c += sizeof(double) * cv;
+            HIPCHK(hipMemcpyAsync(I->D.b, c, sizeof(double) * cv, hipMemcpyDeviceToDevice, B->stream));
+            c += sizeof(double) * cv;
+            HIPCHK(hipMemcpyAsync(I->D.scal, c, sizeof(double) * SC_COUNT, hipMemcpyDeviceToDevice, B->stream));
+            c += sizeof(double) * SC_COUNT;
+            HIPCHK(hipMemcpyAsync(I->D.status, c, sizeof(unsigned), hipMemcpyDeviceToDevice, B->stream));
+            if ((r = upload(I))) return r;
+        }
+        HIPCHK(hipStreamSynchronize(B->stream));
+        return E_OK;
+    };
+    const int wgs[3] = {64, 128, 256}, tiles[3] = {8, 16, 32};
+    double best = 1e300;
+    int bw = B->size.wg_vel, bt = B->size.tile;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int a = 0; a < 3 && !rc; ++a)
+        for (int t = 0; t < 3 && !rc; ++t) {
+            if (ms9) ms9[3 * a + t] = 0.0;
+            if (B->size.nvel % tiles[t]) continue;
+            if ((rc = restore()) || (rc = batch_set_shape(B, wgs[a], tiles[t])) || (rc = batch_push(B)) ||
+                (rc = batch_capture(B)))
+                break;
+            if (hipGraphLaunch(B->exec, B->stream) != hipSuccess) { rc = fail(E_DEVICE, "graph launch failed"); break; }
+            hipEventRecord(e0, B->stream);
+            for (int r = 0; r < reps; ++r) hipGraphLaunch(B->exec, B->stream);
+            hipEventRecord(e1, B->stream);
+            if (hipStreamSynchronize(B->stream) != hipSuccess) { rc = fail(E_DEVICE, "autotune run failed"); break; }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / reps;
+            if (ms9) ms9[3 * a + t] = per;
+            if (per < best) { best = per; bw = wgs[a]; bt = tiles[t]; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    int rr = restore();
+    if (!rc) rc = rr;
+    if (!rc) rc = batch_set_shape(B, bw, bt);
+    release();
+    if (rc) return rc;
+    out2[0] = bw;
+    out2[1] = bt;
     return E_OK;
 }

@@ -231,7 +231,8 @@ __device__ inline void face_data(const Dev& D, bool xf, int i, int j, int k, boo
 // (nx*(ny-1) blocks)
 // ===========================================================================
 template <int NT, bool xf>
-__global__ void __launch_bounds__(NT) k_lo_face(Dev D) {
+__global__ void __launch_bounds__(NT) k_lo_face(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 14];
     __shared__ double tu[LO_MAXAX], tv[LO_MAXAX], tmax[2];
     const int nx = D.nx, ny = D.ny, nu = D.nu, nvel = D.nvel;
@@ -402,7 +403,8 @@ __device__ inline void wall_data(const Dev& D, int top, int i, int c, int k, dou
 // multiscale diffuse walls: blocks [0, nx) bottom faces, [nx, 2nx) top faces
 // ===========================================================================
 template <int NT>
-__global__ void __launch_bounds__(NT) k_lo_wall(Dev D) {
+__global__ void __launch_bounds__(NT) k_lo_wall(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 28];
     __shared__ double tu[LO_MAXAX], tv[LO_MAXAX], tmax[2];
     const int nx = D.nx, ny = D.ny, nu = D.nu, nvel = D.nvel, ck = D.gas.ck;
@@ -575,7 +577,8 @@ __global__ void __launch_bounds__(NT) k_lo_wall(Dev D) {
 // with the projected old and new equilibria (ApplyExplicitUpdate)
 // ===========================================================================
 template <int NT>
-__global__ void __launch_bounds__(NT) k_lo_cell(Dev D) {
+__global__ void __launch_bounds__(NT) k_lo_cell(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 14];
     __shared__ double tuo[LO_MAXAX], tvo[LO_MAXAX], tmo[2], tun[LO_MAXAX], tvn[LO_MAXAX], tmn[2];
     const int nx = D.nx, nu = D.nu, nvel = D.nvel, ck = D.gas.ck;
@@ -703,7 +706,8 @@ __global__ void __launch_bounds__(NT) k_lo_cell(Dev D) {
 }
 
 // residual res(1:4) of the step, in fixed order (Write_residuals semantics)
-__global__ void __launch_bounds__(256) k_lo_finish(Dev D) {
+__global__ void __launch_bounds__(256) k_lo_finish(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[16 * 8];
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = threadIdx.x; c < D.ncell; c += 256)
@@ -723,7 +727,8 @@ __global__ void __launch_bounds__(256) k_lo_finish(Dev D) {
 // freestream cold start: every cell holds W = GetConserved(prim) and the
 // Maxwellian of prim projected onto W
 template <int NT>
-__global__ void __launch_bounds__(NT) k_lo_cold_start(Dev D, double rho, double ux, double tn) {
+__global__ void __launch_bounds__(NT) k_lo_cold_start(const Dev* __restrict__ DS, double rho, double ux, double tn) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 14];
     __shared__ double tu[LO_MAXAX], tv[LO_MAXAX], tmax[2];
     const int nu = D.nu, nvel = D.nvel, ck = D.gas.ck;
@@ -784,43 +789,47 @@ __global__ void __launch_bounds__(NT) k_lo_cold_start(Dev D, double rho, double 
 // ===========================================================================
 // launchers
 // ===========================================================================
-extern "C" hipError_t ugks_launch_timestep(const Dev* D, double umax, double vmax, hipStream_t s, Prof* P);
+extern "C" hipError_t ugks_launch_timestep(const LaunchSet* L, hipStream_t s, Prof* P);
 
 namespace {
 template <int NT>
-hipError_t launch_lo_step(const Dev& D, double umax, double vmax, hipStream_t s, Prof* P) {
-    hipError_t e = ugks_launch_timestep(&D, umax, vmax, s, P);
+hipError_t launch_lo_step(const LaunchSet& L, hipStream_t s, Prof* P) {
+    hipError_t e = ugks_launch_timestep(&L, s, P);
     if (e != hipSuccess) return e;
-    UGKS_TIMED(P, K_LO_FACE, s, hipLaunchKernelGGL((k_lo_face<NT, true>), dim3(D.ncell), dim3(NT), 0, s, D));
+    const Dev& D = *L.size;
+    const int n = L.n;
+    UGKS_TIMED(P, K_LO_FACE, s, hipLaunchKernelGGL((k_lo_face<NT, true>), dim3(D.ncell, 1, n), dim3(NT), 0, s, L.dev));
     UGKS_TIMED(P, K_LO_FACE, s,
-               hipLaunchKernelGGL((k_lo_face<NT, false>), dim3(D.nx * (D.ny - 1)), dim3(NT), 0, s, D));
-    UGKS_TIMED(P, K_LO_WALL, s, hipLaunchKernelGGL(k_lo_wall<NT>, dim3(2 * D.nx), dim3(NT), 0, s, D));
-    UGKS_TIMED(P, K_LO_CELL, s, hipLaunchKernelGGL(k_lo_cell<NT>, dim3(D.ncell), dim3(NT), 0, s, D));
-    UGKS_TIMED(P, K_LO_FINISH, s, hipLaunchKernelGGL(k_lo_finish, dim3(1), dim3(256), 0, s, D));
+               hipLaunchKernelGGL((k_lo_face<NT, false>), dim3(D.nx * (D.ny - 1), 1, n), dim3(NT), 0, s, L.dev));
+    UGKS_TIMED(P, K_LO_WALL, s, hipLaunchKernelGGL(k_lo_wall<NT>, dim3(2 * D.nx, 1, n), dim3(NT), 0, s, L.dev));
+    UGKS_TIMED(P, K_LO_CELL, s, hipLaunchKernelGGL(k_lo_cell<NT>, dim3(D.ncell, 1, n), dim3(NT), 0, s, L.dev));
+    UGKS_TIMED(P, K_LO_FINISH, s, hipLaunchKernelGGL(k_lo_finish, dim3(1, 1, n), dim3(256), 0, s, L.dev));
     return hipGetLastError();
 }
 }  // namespace
 
-// One step of the low-order Explicit entry; the proposal goes to Wn/hn/bn.
-// wg: workgroup size of the face, wall and cell kernels (64 .. 1024, at most
-// Nvel; each thread loops over the velocities k = t, t + wg, ...).
-extern "C" hipError_t ugks_launch_lo_step(const Dev* Dp, int wg, double umax, double vmax, hipStream_t s, Prof* P) {
-    const Dev& D = *Dp;
-    if (wg > D.nvel || D.nu + D.nv > wg) return hipErrorInvalidValue;
+// One step of the low-order Explicit entry (one instance, L.n == 1: the face
+// and cell kernels map workgroups to faces XCD-contiguously over the grid);
+// the proposal goes to Wn/hn/bn.  wg: workgroup size of the face, wall and
+// cell kernels (64 .. 1024, at most Nvel; each thread loops over the
+// velocities k = t, t + wg, ...).
+extern "C" hipError_t ugks_launch_lo_step(const LaunchSet* L, int wg, hipStream_t s, Prof* P) {
+    const Dev& D = *L->size;
+    if (L->n != 1 || wg > D.nvel || D.nu + D.nv > wg) return hipErrorInvalidValue;
     switch (wg) {
-    case 64: return launch_lo_step<64>(D, umax, vmax, s, P);
-    case 128: return launch_lo_step<128>(D, umax, vmax, s, P);
-    case 256: return launch_lo_step<256>(D, umax, vmax, s, P);
-    case 512: return launch_lo_step<512>(D, umax, vmax, s, P);
-    case 1024: return launch_lo_step<1024>(D, umax, vmax, s, P);
+    case 64: return launch_lo_step<64>(*L, s, P);
+    case 128: return launch_lo_step<128>(*L, s, P);
+    case 256: return launch_lo_step<256>(*L, s, P);
+    case 512: return launch_lo_step<512>(*L, s, P);
+    case 1024: return launch_lo_step<1024>(*L, s, P);
     default: return hipErrorInvalidValue;
     }
 }
 
-extern "C" hipError_t ugks_launch_lo_cold_start(const Dev* Dp, double rho, double ux, double tn, hipStream_t s) {
-    const Dev& D = *Dp;
+extern "C" hipError_t ugks_launch_lo_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s) {
+    const Dev& D = *L->size;
     if (D.nu + D.nv > 256) return hipErrorInvalidValue;
     const int nb = D.ncell < 1024 ? D.ncell : 1024;
-    hipLaunchKernelGGL(k_lo_cold_start<256>, dim3(nb), dim3(256), 0, s, D, rho, ux, tn);
+    hipLaunchKernelGGL(k_lo_cold_start<256>, dim3(nb, 1, L->n), dim3(256), 0, s, L->dev, rho, ux, tn);
     return hipGetLastError();
 }

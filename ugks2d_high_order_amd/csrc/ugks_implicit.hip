@@ -52,7 +52,8 @@ __device__ inline double spectral_radius(const Gas& g, const double wi[4], const
 }
 
 // spectral radii of all faces (they depend on W only)
-__global__ void __launch_bounds__(256) k_macro_gamma(Dev D) {
+__global__ void __launch_bounds__(256) k_macro_gamma(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = D.nx, ny = D.ny;
     if (t < D.ncell) {   // east face of cell t
@@ -136,7 +137,8 @@ __device__ inline bool relax_macro(const Dev& D, int i, int j, double dtp) {
 
 // one workgroup: 2 x (forward + backward) hyperplane sweeps, then the
 // predicted macro state (alpha must stay 1 under the closed-Couette policy)
-__global__ void __launch_bounds__(256) k_macro_pr2(Dev D) {
+__global__ void __launch_bounds__(256) k_macro_pr2(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     const int nx = D.nx, ny = D.ny, nd = nx + ny - 1;
     const double dtp = D.scal[SC_DTP];
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x)
@@ -214,9 +216,11 @@ __device__ inline void gram_full(const double G[10], double M[16]) {
 }
 
 template <int KPT, int NT>
-__global__ void __launch_bounds__(NT) k_predict(Dev D) {
+__global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     const int c = blockIdx.x, nvel = D.nvel;
+    if (c >= D.ncell) return;
     const Gas& g = D.gas;
     double u[KPT], v[KPT], w[KPT], h[KPT], b[KPT];
 #pragma unroll
@@ -324,7 +328,8 @@ struct MicroCoef {
 // forward sweep: delta = (R + upwind new neighbours)/D; neighbours across the
 // x seam and on the old side are zero (reset), the wall closure of the
 // forward sweep is frozen from a zero increment
-__global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
+__global__ void __launch_bounds__(256) k_sweep_fwd(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel;
     const int lane = threadIdx.x % T, slot = threadIdx.x / T, nslot = blockDim.x / T;
@@ -392,7 +397,8 @@ __global__ void __launch_bounds__(256) k_sweep_fwd(Dev D) {
 
 // backward sweep with the apply step: f' = f + delta (alpha = 1), per-tile
 // moment partials of f' for the macro recovery
-__global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
+__global__ void __launch_bounds__(256) k_sweep_bwd(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T] + [2][nx]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
     const int lane = threadIdx.x % T, slot = threadIdx.x / T, nslot = blockDim.x / T;
@@ -494,7 +500,8 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(Dev D) {
 }
 
 // reflection normalisers of the frozen wall closure (depend on the table only)
-__global__ void __launch_bounds__(256) k_wall_reflection(Dev D) {
+__global__ void __launch_bounds__(256) k_wall_reflection(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[16 * 4];
     double s[4] = {0, 0, 0, 0};
     for (int k = threadIdx.x; k < D.nvel; k += blockDim.x) {
@@ -519,7 +526,8 @@ __global__ void __launch_bounds__(256) k_wall_reflection(Dev D) {
 }
 
 // macro recovery of the proposal, admissibility, mass of the proposal
-__global__ void __launch_bounds__(256) k_recover(Dev D) {
+__global__ void __launch_bounds__(256) k_recover(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.ncell) return;
     double m[4] = {0, 0, 0, 0};
@@ -529,7 +537,8 @@ __global__ void __launch_bounds__(256) k_recover(Dev D) {
     if (!physical_w(m)) atomicOr(D.status, (unsigned)ST_MICRO_ALPHA);
 }
 
-__global__ void __launch_bounds__(256) k_mass(Dev D, int rescale) {
+__global__ void __launch_bounds__(256) k_mass(const Dev* __restrict__ DS, int rescale) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[16 * 1];
     double s[1] = {0.0}, s0[1] = {0.0};
     const double area = D.dx * D.dy;
@@ -546,7 +555,8 @@ __global__ void __launch_bounds__(256) k_mass(Dev D, int rescale) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_rescale(Dev D) {
+__global__ void __launch_bounds__(256) k_rescale(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     const size_t n = (size_t)D.ncell * D.nvel;
     const double c = D.scal[SC_SCALE];
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -558,7 +568,8 @@ __global__ void __launch_bounds__(256) k_rescale(Dev D) {
         D.Wn[t] = c * D.Wn[t];
 }
 
-__global__ void __launch_bounds__(256) k_mass_after(Dev D) {
+__global__ void __launch_bounds__(256) k_mass_after(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[16 * 1];
     double s[1] = {0.0};
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) s[0] += D.dx * D.dy * D.Wn[4 * c];
@@ -566,29 +577,79 @@ __global__ void __launch_bounds__(256) k_mass_after(Dev D) {
     if (threadIdx.x == 0) D.scal[SC_MASS_AFTER] = s[0];
 }
 
-extern "C" hipError_t ugks_launch_wall_reflection(const Dev* Dp, hipStream_t s, Prof* P) {
-    UGKS_TIMED(P, K_WALL_REFLECTION, s, hipLaunchKernelGGL(k_wall_reflection, dim3(1), dim3(256), 0, s, *Dp));
+extern "C" hipError_t ugks_launch_wall_reflection(const LaunchSet* L, hipStream_t s, Prof* P) {
+    UGKS_TIMED(P, K_WALL_REFLECTION, s,
+               hipLaunchKernelGGL(k_wall_reflection, dim3(1, 1, L->n), dim3(256), 0, s, L->dev));
     return hipGetLastError();
 }
 
-extern "C" hipError_t ugks_launch_implicit(const Dev* Dp, int rescale, hipStream_t s, Prof* P) {
-    const Dev& D = *Dp;
+// The implicit update of every instance in L (the operator of the current
+// state must be in place).  The proposal goes to Wn/hn/bn.
+extern "C" hipError_t ugks_launch_implicit(const LaunchSet* L, int rescale, hipStream_t s, Prof* P) {
+    const Dev& D = *L->size;
+    const int n = L->n;
+    const Dev* DS = L->dev;
     int ng = (D.ncell + (D.ny + 1) * D.nx + 255) / 256;
-    UGKS_TIMED(P, K_MACRO_GAMMA, s, hipLaunchKernelGGL(k_macro_gamma, dim3(ng), dim3(256), 0, s, D));
-    UGKS_TIMED(P, K_MACRO_PR2, s, hipLaunchKernelGGL(k_macro_pr2, dim3(1), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_MACRO_GAMMA, s, hipLaunchKernelGGL(k_macro_gamma, dim3(ng, 1, n), dim3(256), 0, s, DS));
+    UGKS_TIMED(P, K_MACRO_PR2, s, hipLaunchKernelGGL(k_macro_pr2, dim3(1, 1, n), dim3(256), 0, s, DS));
     switch (D.nvel) {
-    case 256: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 64>), dim3(D.ncell), dim3(64), 0, s, D)); break;
-    case 1024: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 256>), dim3(D.ncell), dim3(256), 0, s, D)); break;
-    case 4096: UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<8, 512>), dim3(D.ncell), dim3(512), 0, s, D)); break;
+    case 256:
+        UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 64>), dim3(D.ncell, 1, n), dim3(64), 0, s, DS));
+        break;
+    case 1024:
+        UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 256>), dim3(D.ncell, 1, n), dim3(256), 0, s, DS));
+        break;
+    case 4096:
+        UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<8, 512>), dim3(D.ncell, 1, n), dim3(512), 0, s, DS));
+        break;
     default: return hipErrorInvalidValue;
     }
     size_t shf = sizeof(double) * 4 * D.nx * D.tile;
     size_t shb = shf + sizeof(double) * 2 * D.nx;
-    UGKS_TIMED(P, K_SWEEP_FWD, s, hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles), dim3(256), shf, s, D));
-    UGKS_TIMED(P, K_SWEEP_BWD, s, hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles), dim3(256), shb, s, D));
-    UGKS_TIMED(P, K_RECOVER, s, hipLaunchKernelGGL(k_recover, dim3((D.ncell + 255) / 256), dim3(256), 0, s, D));
-    UGKS_TIMED(P, K_MASS, s, hipLaunchKernelGGL(k_mass, dim3(1), dim3(256), 0, s, D, rescale));
-    UGKS_TIMED(P, K_RESCALE, s, hipLaunchKernelGGL(k_rescale, dim3(1024), dim3(256), 0, s, D));
-    UGKS_TIMED(P, K_MASS_AFTER, s, hipLaunchKernelGGL(k_mass_after, dim3(1), dim3(256), 0, s, D));
+    UGKS_TIMED(P, K_SWEEP_FWD, s, hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles, 1, n), dim3(256), shf, s, DS));
+    UGKS_TIMED(P, K_SWEEP_BWD, s, hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles, 1, n), dim3(256), shb, s, DS));
+    UGKS_TIMED(P, K_RECOVER, s, hipLaunchKernelGGL(k_recover, dim3((D.ncell + 255) / 256, 1, n), dim3(256), 0, s, DS));
+    UGKS_TIMED(P, K_MASS, s, hipLaunchKernelGGL(k_mass, dim3(1, 1, n), dim3(256), 0, s, DS, rescale));
+    UGKS_TIMED(P, K_RESCALE, s, hipLaunchKernelGGL(k_rescale, dim3(1024, 1, n), dim3(256), 0, s, DS));
+    UGKS_TIMED(P, K_MASS_AFTER, s, hipLaunchKernelGGL(k_mass_after, dim3(1, 1, n), dim3(256), 0, s, DS));
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// batched instances: device-side acceptance of an iteration
+// ===========================================================================
+// An instance whose status word is clear swaps its state and proposal buffers
+// (the pointer swap the single-instance host does after an accepted update)
+// and counts the iteration; any status bit leaves its accepted state as it
+// is, so a rejected instance keeps re-evaluating the same state.  One thread
+// per instance.
+__global__ void __launch_bounds__(64) k_batch_accept(Dev* DS, int n, int* count, unsigned* status_out) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= n) return;
+    Dev& D = DS[z];
+    const unsigned st = *D.status;
+    status_out[z] = st;
+    if (st == 0) {
+        double* t;
+        t = D.W; D.W = D.Wn; D.Wn = t;
+        t = D.h; D.h = D.hn; D.hn = t;
+        t = D.b; D.b = D.bn; D.bn = t;
+        count[z] += 1;
+    }
+}
+
+extern "C" hipError_t ugks_launch_batch_accept(Dev* DS, int n, int* count, unsigned* status_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_accept, dim3((n + 63) / 64), dim3(64), 0, s, DS, n, count, status_out);
+    return hipGetLastError();
+}
+
+// zero n ints (a kernel, not a memset node, so that it may sit in a graph)
+__global__ void __launch_bounds__(64) k_zero_ints(int* p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+extern "C" hipError_t ugks_launch_zero_ints(int* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_ints, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
     return hipGetLastError();
 }

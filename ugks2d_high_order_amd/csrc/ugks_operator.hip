@@ -19,7 +19,9 @@ __constant__ Consts KC;
 // ===========================================================================
 // time step (one workgroup)
 // ===========================================================================
-__global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vmax) {
+__global__ void __launch_bounds__(256) k_timestep(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const double umax = D.umax, vmax = D.vmax;
     __shared__ double lds[16];
     double tmin = 1e300;
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
@@ -43,9 +45,13 @@ __global__ void __launch_bounds__(256) k_timestep(Dev D, double umax, double vma
 // ===========================================================================
 // One workgroup row per cell row (blockIdx.y = j), so the wall-stencil choice
 // is uniform over the workgroup and the near-wall branch never diverges.
-__global__ void __launch_bounds__(256) k_wrecon(Dev D, const double* Wsrc, double* ws_out, double* wv_out) {
+__global__ void __launch_bounds__(256) k_wrecon(const Dev* __restrict__ DS, int proposal) {
+    const Dev& D = DS[blockIdx.z];
+    const double* Wsrc = proposal ? D.Wn : D.W;
+    double* ws_out = D.ws;
+    double* wv_out = D.wv;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 4 * D.nx) return;
+    if (t >= 4 * D.nx || (int)blockIdx.y >= D.ny) return;
     const int j = blockIdx.y, i = t >> 2, m = t & 3, c = j * D.nx + i;
     int rb, sh;
     bool near;
@@ -141,10 +147,12 @@ __device__ inline void store_side(double* rec, size_t nv, int off, double theta,
 
 // Two lanes per (cell, velocity): the even lane reconstructs h, the odd lane
 // b; the positivity factor is the minimum of the pair, exchanged through DPP.
-__global__ void __launch_bounds__(128) k_recon(Dev D) {
+__global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = t >> 1, fld = t & 1;
     const int c = blockIdx.y;
+    if (c >= D.ncell) return;
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const size_t nv = (size_t)nvel;
     const int i = c % nx, j = c / nx;
@@ -462,12 +470,14 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
 }
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_face_flux(Dev D) {
+__global__ void __launch_bounds__(NT) k_face_flux(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     UGKS_DYN_SHARED(double, qh);
     const int nx = D.nx, nvel = D.nvel;
     const int f = blockIdx.x;
     const int nfx = D.ncell;
+    if (f >= nfx + nx * (D.ny - 1)) return;
     int axis, cl, cr;
     const double* rec;
     double *oh, *ob, *ow;
@@ -615,11 +625,13 @@ __device__ inline void wall_gin(const double mt[5], const double an[4], const do
 }
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_wall_flux(Dev D) {
+__global__ void __launch_bounds__(NT) k_wall_flux(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     UGKS_DYN_SHARED(double, mhs);
     const int nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
     const size_t nv = (size_t)nvel;
+    if ((int)blockIdx.x >= 2 * nx) return;
     const int top = blockIdx.x >= nx ? 1 : 0, i = blockIdx.x % nx;
     const int side = top ? 1 : -1;
     const int c = top ? (ny - 1) * nx + i : i;
@@ -823,7 +835,8 @@ __device__ inline void target_kernels(const double* node, int ck, double u, doub
 }
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cell(Dev D) {
+__global__ void __launch_bounds__(NT) k_cell(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     __shared__ double node[9 * VN_LEN];
     UGKS_DYN_SHARED(double, dyn);
@@ -831,6 +844,7 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
     double* amh = dyn;
     double* amb = dyn + nvel;
     const int c = blockIdx.x, i = c % nx, j = c / nx;
+    if (c >= D.ncell) return;
     const double* Wc = D.W + 4 * c;
     double W[4] = {Wc[0], Wc[1], Wc[2], Wc[3]};
     unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds);
@@ -915,9 +929,11 @@ __global__ void __launch_bounds__(NT) k_cell(Dev D) {
 // (src/mod_S2NativeAdapter.f90:877-905), and its conservation gate (:925-948)
 // ===========================================================================
 // streamed state: Wn = W + flux residual, hs/bs (into rh/rb) = h/b + flux residual
-__global__ void __launch_bounds__(256) k_cn_stream(Dev D) {
+__global__ void __launch_bounds__(256) k_cn_stream(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y, nx = D.nx, nvel = D.nvel;
+    if (c >= D.ncell) return;
     const int j = c / nx, i = c % nx, cw = j * nx + (i == 0 ? nx - 1 : i - 1);
     const size_t fs = (size_t)j * nx + i, fn = (size_t)(j + 1) * nx + i;
     const double area = D.dx * D.dy, ainv = 1.0 / area;
@@ -937,11 +953,13 @@ __global__ void __launch_bounds__(256) k_cn_stream(Dev D) {
 
 // volume targets of the state W into tg (first step of an S2CN2 run)
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cn_targets(Dev D) {
+__global__ void __launch_bounds__(NT) k_cn_targets(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     __shared__ double node[9 * VN_LEN];
     UGKS_DYN_SHARED(double, dyn);
     const int nvel = D.nvel, ck = D.gas.ck, c = blockIdx.x;
+    if (c >= D.ncell) return;
     double* amh = dyn;
     double* amb = dyn + nvel;
     double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
@@ -970,11 +988,13 @@ __device__ inline void heat_add(double q[2], double w, double cx, double cy, dou
 // Crank-Nicolson Shakhov collision of one cell: new targets at Wn (written to
 // tgn), the 2x2 heat-flux solve and hn, bn; per-cell gate quantities into cnpart.
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cn_cell(Dev D) {
+__global__ void __launch_bounds__(NT) k_cn_cell(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     __shared__ double node[9 * VN_LEN];
     UGKS_DYN_SHARED(double, dyn);
     const int nvel = D.nvel, ck = D.gas.ck, c = blockIdx.x;
+    if (c >= D.ncell) return;
     double* amh = dyn;
     double* amb = dyn + nvel;
     const double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
@@ -1069,7 +1089,9 @@ __global__ void __launch_bounds__(NT) k_cn_cell(Dev D) {
 }
 
 // gates of the step and the conservation ledger (one workgroup)
-__global__ void __launch_bounds__(256) k_cn_finish(Dev D, double umax, double vmax) {
+__global__ void __launch_bounds__(256) k_cn_finish(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const double umax = D.umax, vmax = D.vmax;
     __shared__ double lds[16 * 9];
     const int nc = D.ncell, nx = D.nx, ny = D.ny;
     double mx[7] = {0.0, 0.0, 0.0, -1e308, -1e308, -1e308, -1e308};   // cn, mm, 2beta, -hmin, hmax, -bmin, bmax
@@ -1117,7 +1139,9 @@ __global__ void __launch_bounds__(256) k_cn_finish(Dev D, double umax, double vm
 // ===========================================================================
 // residual diagnostics (one workgroup): sums of the per-cell partials
 // ===========================================================================
-__global__ void __launch_bounds__(256) k_norms(Dev D, double residual_tol) {
+__global__ void __launch_bounds__(256) k_norms(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const double residual_tol = D.residual_tol;
     __shared__ double lds[16 * 14];
     double s[14];
     for (int q = 0; q < 14; ++q) s[q] = 0.0;
@@ -1175,7 +1199,8 @@ __global__ void __launch_bounds__(256) k_norms(Dev D, double residual_tol) {
 // cold start: the rest state's discrete conservative equilibrium in every cell
 // ===========================================================================
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux, double tn) {
+__global__ void __launch_bounds__(NT) k_cold_start(const Dev* __restrict__ DS, double rho, double ux, double tn) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     UGKS_DYN_SHARED(double, qh);
     const double lam = 1.0 / tn;
@@ -1199,9 +1224,11 @@ __global__ void __launch_bounds__(NT) k_cold_start(Dev D, double rho, double ux,
 // observables of the current state: rho, U, V, T, p, stresses, heat flux
 // ===========================================================================
 template <int NT>
-__global__ void __launch_bounds__(NT) k_observables(Dev D, double* out) {
+__global__ void __launch_bounds__(NT) k_observables(const Dev* __restrict__ DS, double* out) {
+    const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     const int c = blockIdx.x, nvel = D.nvel;
+    if (c >= D.ncell) return;
     double prim[4];
     primary(D.gas.gamma, D.W + 4 * c, prim);
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1241,105 +1268,111 @@ static hipError_t allow_lds(K kernel, size_t bytes, size_t* done) {
     return e;
 }
 
+// Cold start and observables act on one instance (L.n == 1).
 template <int NT>
-static hipError_t launch_cold_start(const Dev& D, double rho, double ux, double tn, hipStream_t s) {
+static hipError_t launch_cold_start(const LaunchSet& L, double rho, double ux, double tn, hipStream_t s) {
     static size_t lds_cold = 0;
+    const Dev& D = *L.size;
     const size_t q1 = sizeof(double) * D.nvel;
     hipError_t e = allow_lds(k_cold_start<NT>, q1, &lds_cold);
     if (e != hipSuccess) return e;
     int nb = D.ncell < 1024 ? D.ncell : 1024;
-    hipLaunchKernelGGL(k_cold_start<NT>, dim3(nb), dim3(NT), q1, s, D, rho, ux, tn);
+    hipLaunchKernelGGL(k_cold_start<NT>, dim3(nb, 1, L.n), dim3(NT), q1, s, L.dev, rho, ux, tn);
     return hipGetLastError();
 }
 
-extern "C" hipError_t ugks_launch_cold_start(const Dev* Dp, double rho, double ux, double tn, hipStream_t s) {
-    switch (Dp->wg_vel) {
-    case 64: return launch_cold_start<64>(*Dp, rho, ux, tn, s);
-    case 128: return launch_cold_start<128>(*Dp, rho, ux, tn, s);
-    case 256: return launch_cold_start<256>(*Dp, rho, ux, tn, s);
+extern "C" hipError_t ugks_launch_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s) {
+    switch (L->size->wg_vel) {
+    case 64: return launch_cold_start<64>(*L, rho, ux, tn, s);
+    case 128: return launch_cold_start<128>(*L, rho, ux, tn, s);
+    case 256: return launch_cold_start<256>(*L, rho, ux, tn, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-extern "C" hipError_t ugks_launch_observables(const Dev* Dp, double* out, hipStream_t s) {
-    const Dev& D = *Dp;
-    hipLaunchKernelGGL(k_observables<256>, dim3(D.ncell), dim3(256), 0, s, D, out);
+extern "C" hipError_t ugks_launch_observables(const LaunchSet* L, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_observables<256>, dim3(L->size->ncell, 1, L->n), dim3(256), 0, s, L->dev, out);
     return hipGetLastError();
 }
 
 // face, wall and cell kernels with NT-thread workgroups
 template <int NT>
-static hipError_t launch_closure_kernels(const Dev& D, hipStream_t s, Prof* P) {
+static hipError_t launch_closure_kernels(const LaunchSet& L, hipStream_t s, Prof* P) {
     static size_t lds_face = 0, lds_wall = 0, lds_cell = 0;
+    const Dev& D = *L.size;
     const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
     hipError_t e;
     if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
     if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
     if ((e = allow_lds(k_cell<NT>, q2, &lds_cell)) != hipSuccess) return e;
     UGKS_TIMED(P, K_FACE_FLUX, s,
-               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), q1, s, D));
-    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx), dim3(NT), q1, s, D));
-    UGKS_TIMED(P, K_CELL, s, hipLaunchKernelGGL(k_cell<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
+               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1), 1, L.n), dim3(NT), q1, s, L.dev));
+    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx, 1, L.n), dim3(NT), q1, s, L.dev));
+    UGKS_TIMED(P, K_CELL, s, hipLaunchKernelGGL(k_cell<NT>, dim3(D.ncell, 1, L.n), dim3(NT), q2, s, L.dev));
     return hipSuccess;
 }
 
 template <int NT>
-static hipError_t launch_cn(const Dev& D, bool build_old, double umax, double vmax, hipStream_t s, Prof* P) {
+static hipError_t launch_cn(const LaunchSet& L, bool build_old, hipStream_t s, Prof* P) {
     static size_t lds_t = 0, lds_c = 0, lds_face = 0, lds_wall = 0;
+    const Dev& D = *L.size;
     const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
     hipError_t e;
     if ((e = allow_lds(k_cn_targets<NT>, q2, &lds_t)) != hipSuccess) return e;
     if ((e = allow_lds(k_cn_cell<NT>, q2, &lds_c)) != hipSuccess) return e;
     if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
     if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
-    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
-    const dim3 gw((4 * D.nx + 255) / 256, D.ny);
-    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, D, D.W, D.ws, D.wv));
-    if (build_old) UGKS_TIMED(P, K_CN_TARGETS, s, hipLaunchKernelGGL(k_cn_targets<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
-    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell), dim3(128), 0, s, D));
+    const int n = L.n;
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1, 1, n), dim3(256), 0, s, L.dev));
+    const dim3 gw((4 * D.nx + 255) / 256, D.ny, n);
+    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, L.dev, 0));
+    if (build_old)
+        UGKS_TIMED(P, K_CN_TARGETS, s, hipLaunchKernelGGL(k_cn_targets<NT>, dim3(D.ncell, 1, n), dim3(NT), q2, s, L.dev));
+    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
     UGKS_TIMED(P, K_FACE_FLUX, s,
-               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1)), dim3(NT), q1, s, D));
-    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx), dim3(NT), q1, s, D));
+               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1), 1, n), dim3(NT), q1, s, L.dev));
+    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx, 1, n), dim3(NT), q1, s, L.dev));
     UGKS_TIMED(P, K_CN_STREAM, s,
-               hipLaunchKernelGGL(k_cn_stream, dim3((D.nvel + 255) / 256, D.ncell), dim3(256), 0, s, D));
-    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, D, D.Wn, D.ws, D.wv));
-    UGKS_TIMED(P, K_CN_CELL, s, hipLaunchKernelGGL(k_cn_cell<NT>, dim3(D.ncell), dim3(NT), q2, s, D));
-    UGKS_TIMED(P, K_CN_FINISH, s, hipLaunchKernelGGL(k_cn_finish, dim3(1), dim3(256), 0, s, D, umax, vmax));
+               hipLaunchKernelGGL(k_cn_stream, dim3((D.nvel + 255) / 256, D.ncell, n), dim3(256), 0, s, L.dev));
+    UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, L.dev, 1));
+    UGKS_TIMED(P, K_CN_CELL, s, hipLaunchKernelGGL(k_cn_cell<NT>, dim3(D.ncell, 1, n), dim3(NT), q2, s, L.dev));
+    UGKS_TIMED(P, K_CN_FINISH, s, hipLaunchKernelGGL(k_cn_finish, dim3(1, 1, n), dim3(256), 0, s, L.dev));
     return hipGetLastError();
 }
 
 // One S2CN2 step on the current state (build_old: first build the volume
 // targets of the state itself).  The proposal goes to Wn/hn/bn and tgn.
-extern "C" hipError_t ugks_launch_cn(const Dev* Dp, int build_old, double umax, double vmax, hipStream_t s,
-                                     Prof* P) {
-    switch (Dp->wg_vel) {
-    case 64: return launch_cn<64>(*Dp, build_old != 0, umax, vmax, s, P);
-    case 128: return launch_cn<128>(*Dp, build_old != 0, umax, vmax, s, P);
-    case 256: return launch_cn<256>(*Dp, build_old != 0, umax, vmax, s, P);
+extern "C" hipError_t ugks_launch_cn(const LaunchSet* L, int build_old, hipStream_t s, Prof* P) {
+    switch (L->size->wg_vel) {
+    case 64: return launch_cn<64>(*L, build_old != 0, s, P);
+    case 128: return launch_cn<128>(*L, build_old != 0, s, P);
+    case 256: return launch_cn<256>(*L, build_old != 0, s, P);
     default: return hipErrorInvalidValue;
     }
 }
 
-extern "C" hipError_t ugks_launch_timestep(const Dev* Dp, double umax, double vmax, hipStream_t s, Prof* P) {
-    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, *Dp, umax, vmax));
+extern "C" hipError_t ugks_launch_timestep(const LaunchSet* L, hipStream_t s, Prof* P) {
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1, 1, L->n), dim3(256), 0, s, L->dev));
     return hipGetLastError();
 }
 
-extern "C" hipError_t ugks_launch_operator(const Dev* Dp, double umax, double vmax, double residual_tol,
-                                           hipStream_t s, Prof* P) {
-    const Dev& D = *Dp;
-    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, s, D, umax, vmax));
+// The physical operator of the current state of every instance in L.
+extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Prof* P) {
+    const Dev& D = *L->size;
+    const int n = L->n;
+    UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1, 1, n), dim3(256), 0, s, L->dev));
     UGKS_TIMED(P, K_WRECON, s,
-               hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny), dim3(256), 0, s, D, D.W, D.ws, D.wv));
-    UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell), dim3(128), 0, s, D));
+               hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny, n), dim3(256), 0, s, L->dev, 0));
+    UGKS_TIMED(P, K_RECON, s,
+               hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L->dev));
     hipError_t e;
     switch (D.wg_vel) {
-    case 64: e = launch_closure_kernels<64>(D, s, P); break;
-    case 128: e = launch_closure_kernels<128>(D, s, P); break;
-    case 256: e = launch_closure_kernels<256>(D, s, P); break;
+    case 64: e = launch_closure_kernels<64>(*L, s, P); break;
+    case 128: e = launch_closure_kernels<128>(*L, s, P); break;
+    case 256: e = launch_closure_kernels<256>(*L, s, P); break;
     default: e = hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
-    UGKS_TIMED(P, K_NORMS, s, hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, s, D, residual_tol));
+    UGKS_TIMED(P, K_NORMS, s, hipLaunchKernelGGL(k_norms, dim3(1, 1, n), dim3(256), 0, s, L->dev));
     return hipGetLastError();
 }

@@ -31,9 +31,15 @@ enum ScalarSlot {
 // unit length), b value, b derivatives.
 enum { REC_ROWS = 18 };
 
+// One instance's descriptor.  Kernels receive a device array of them and
+// work on DS[blockIdx.z]: a plain launch has one instance, a batched launch
+// covers several instances (grid z) with grid x/y sized for the largest one,
+// and workgroups beyond an instance's own range return at once.
 struct Dev {
     int nx, ny, nu, nv, nvel, ncell;
     double dx, dy, cfl, ncfl;
+    double umax, vmax;        // largest u and v of the velocity table (TimeStep)
+    double residual_tol;      // Implicit_ResidualTol (convergence gate of the norms)
     Gas gas;
     double wall_ux[2], wall_tn[2];
     int macro_sweeps;
@@ -68,4 +74,14 @@ struct Dev {
     double *cnpart;       // [ncell][8] per-cell CN diagnostics
     double *scal;         // ScalarSlot
     unsigned *status;
+};
+
+// Host side of a launch: the descriptor that sizes the grids (for a batch,
+// the largest instance; all instances share nx and the velocity grid), the
+// device array of descriptors the kernels read, and the number of instances
+// (grid z).
+struct LaunchSet {
+    const Dev* size;   // host memory
+    const Dev* dev;    // device memory, n descriptors
+    int n;
 };
