@@ -16,6 +16,7 @@
 
 #include "ugks_state.h"
 #include "ugks_prof.h"
+#include "ugks_recon.h"
 
 extern "C" hipError_t ugks_set_consts(const Consts* k);
 extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Prof* P);
@@ -232,6 +233,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
         (rc = dmalloc(I, &D.xfw, 4 * nc)) || (rc = dmalloc(I, &D.yfw, 4 * (ny + 1) * nx)) ||
         (rc = dmalloc(I, &D.xf, nc * REC_ROWS * nvel)) || (rc = dmalloc(I, &D.yf, nx * (ny - 1) * REC_ROWS * nvel)) ||
         (rc = dmalloc(I, &D.wb, nx * REC_ROWS * nvel)) || (rc = dmalloc(I, &D.wt, nx * REC_ROWS * nvel)) ||
+        (rc = dmalloc(I, &D.lines, nc * LINE_ROWS * 2 * nvel)) ||
         (rc = dmalloc(I, &D.ws, nc * 4 * 24)) || (rc = dmalloc(I, &D.wv, nc * 4 * 9)) ||
         (rc = dmalloc(I, &D.cellpart, nc * 16)) || (rc = dmalloc(I, &D.gx, nc)) ||
         (rc = dmalloc(I, &D.gy, (ny + 1) * nx)) || (rc = dmalloc(I, &D.tau_t, nc)) ||

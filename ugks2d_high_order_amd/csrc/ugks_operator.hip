@@ -85,20 +85,28 @@ struct SideOut {
     double d[3][3];   // node x {value, dn, dt}
 };
 
-__device__ inline void recon_field(const double* F, int nx, int nvel, int i, int rb, bool near, int sh, int k,
-                                   int xs, int ys, int wsd, double fval[4][3], double vol[3][3], SideOut& xo,
+// Second stage of one field at cell (i, j): the FirstLayer gathered from the
+// lines of the column stencil (rows rb..rb+4 at column i) and of the row
+// stencil (columns i-2..i+2 of row j); t = 2k + field within the cell.
+__device__ inline void recon_field(const double* lines, int nx, int nvel, int i, int j, int rb, bool near, int sh,
+                                   int t, int xs, int ys, int wsd, double fval[4][3], double vol[3][3], SideOut& xo,
                                    SideOut& yo, SideOut& wo) {
-    double blk[5][5];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            int col = i + s - 2;
-            col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
-            blk[r][s] = F[(size_t)((rb + r) * nx + col) * nvel + k];
-        }
+    const size_t row = 2 * (size_t)nvel;
     FirstLayer L;
-    first_layer(KC, blk, near, sh, L);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double* p = lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
+        L.xv[r][0] = p[0]; L.xv[r][1] = p[row]; L.xd[r][0] = p[2 * row]; L.xd[r][1] = p[3 * row];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) L.rows[r][a] = p[(4 + a) * row];
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        int col = i + s - 2;
+        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+        const double* p = lines + (size_t)(j * nx + col) * LINE_ROWS * row + t;
+        L.yv[s][0] = p[7 * row]; L.yv[s][1] = p[8 * row]; L.yd[s][0] = p[9 * row]; L.yd[s][1] = p[10 * row];
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         double d[3][3];
@@ -145,8 +153,42 @@ __device__ inline void store_side(double* rec, size_t nv, int off, double theta,
     }
 }
 
-// Two lanes per (cell, velocity): the even lane reconstructs h, the odd lane
-// b; the positivity factor is the minimum of the pair, exchanged through DPP.
+// First stage: the x and y lines of every (cell, velocity, field); two
+// lanes per (cell, velocity), the even lane for h and the odd lane for b.
+__global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t >> 1, fld = t & 1;
+    const int c = blockIdx.y;
+    if (c >= D.ncell) return;
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    const int i = c % nx, j = c / nx;
+    const double* F = fld == 0 ? D.h : D.b;
+    double q[5], xo[7], yo[4];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        int col = i + s - 2;
+        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+        q[s] = F[(size_t)(j * nx + col) * nvel + k];
+    }
+    x_line(KC, q, xo);
+    int rb, sh;
+    bool near;
+    stencil_rows(j, ny, &rb, &near, &sh);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) q[r] = F[(size_t)((rb + r) * nx + i) * nvel + k];
+    y_line(KC, q, near, sh, yo);
+    const size_t row = 2 * (size_t)nvel;
+    double* p = D.lines + (size_t)c * LINE_ROWS * row + t;
+#pragma unroll
+    for (int a = 0; a < 7; ++a) p[a * row] = xo[a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) p[(7 + a) * row] = yo[a];
+}
+
+// Second stage, limiter and face records.  Two lanes per (cell, velocity):
+// the even lane reconstructs h, the odd lane b; the positivity factor is the
+// minimum of the pair, exchanged through DPP.
 __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,7 +209,8 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     double fv[4][3], vol[3][3];
     SideOut xo, yo, wo;
     const double m = F[(size_t)c * nv + k];
-    recon_field(F, nx, nvel, i, rb, near, sh, k, xs, ys, wsd, fv, vol, xo, yo, wo);
+    recon_field(D.lines, nx, nvel, i, j, rb, near, sh, t, xs, ys, wsd, fv,
+                vol, xo, yo, wo);
     double theta = theta_of(m, fv, vol, 1.0);
     theta = fmin(theta, lane_xor1(theta));
     const double m_other = lane_xor1(m);
@@ -1328,6 +1371,8 @@ static hipError_t launch_cn(const LaunchSet& L, bool build_old, hipStream_t s, P
     UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, L.dev, 0));
     if (build_old)
         UGKS_TIMED(P, K_CN_TARGETS, s, hipLaunchKernelGGL(k_cn_targets<NT>, dim3(D.ncell, 1, n), dim3(NT), q2, s, L.dev));
+    UGKS_TIMED(P, K_RECON_LINES, s,
+               hipLaunchKernelGGL(k_recon_lines, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
     UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
     UGKS_TIMED(P, K_FACE_FLUX, s,
                hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1), 1, n), dim3(NT), q1, s, L.dev));
@@ -1363,6 +1408,8 @@ extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Pr
     UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1, 1, n), dim3(256), 0, s, L->dev));
     UGKS_TIMED(P, K_WRECON, s,
                hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny, n), dim3(256), 0, s, L->dev, 0));
+    UGKS_TIMED(P, K_RECON_LINES, s,
+               hipLaunchKernelGGL(k_recon_lines, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L->dev));
     UGKS_TIMED(P, K_RECON, s,
                hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L->dev));
     hipError_t e;

@@ -14,7 +14,7 @@ enum KernelId {
     K_TIMESTEP, K_WRECON, K_RECON, K_FACE_FLUX, K_WALL_FLUX, K_CELL, K_NORMS,
     K_MACRO_GAMMA, K_MACRO_PR2, K_PREDICT, K_SWEEP_FWD, K_SWEEP_BWD, K_RECOVER, K_MASS, K_RESCALE,
     K_MASS_AFTER, K_COLD_START, K_OBSERVABLES, K_WALL_REFLECTION, K_CN_TARGETS, K_CN_STREAM, K_CN_CELL,
-    K_CN_FINISH, K_LO_FACE, K_LO_WALL, K_LO_CELL, K_LO_FINISH, K_COUNT
+    K_CN_FINISH, K_LO_FACE, K_LO_WALL, K_LO_CELL, K_LO_FINISH, K_RECON_LINES, K_COUNT
 };
 
 inline const char* kernel_name(int id) {
@@ -22,7 +22,7 @@ inline const char* kernel_name(int id) {
         "k_timestep", "k_wrecon", "k_recon", "k_face_flux", "k_wall_flux", "k_cell", "k_norms",
         "k_macro_gamma", "k_macro_pr2", "k_predict", "k_sweep_fwd", "k_sweep_bwd", "k_recover", "k_mass",
         "k_rescale", "k_mass_after", "k_cold_start", "k_observables", "k_wall_reflection", "k_cn_targets",
-        "k_cn_stream", "k_cn_cell", "k_cn_finish", "k_lo_face", "k_lo_wall", "k_lo_cell", "k_lo_finish"};
+        "k_cn_stream", "k_cn_cell", "k_cn_finish", "k_lo_face", "k_lo_wall", "k_lo_cell", "k_lo_finish", "k_recon_lines"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 

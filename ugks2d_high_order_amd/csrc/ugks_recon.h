@@ -44,6 +44,40 @@ __device__ inline void first_layer(const Consts& K, const double (&blk)[5][5], b
     }
 }
 
+// The first stage split by cell, for the distributions.  The x line of a
+// cell is its own row's reconstruction centred on it (value and derivative
+// at its two x faces, values at the three Gauss nodes); the y line of a cell
+// is its own column's reconstruction evaluated at its two y faces (the wall
+// quartic of its row's stencil near a wall).  A cell's FirstLayer is the x
+// lines of the five cells of its column stencil and the y lines of the five
+// cells of its row stencil, so each line is computed once instead of once
+// per neighbour.  Line rows: xv(-1/2), xv(+1/2), xd(-1/2), xd(+1/2), three
+// Gauss-node values, then yv(-1/2), yv(+1/2), yd(-1/2), yd(+1/2).
+enum { LINE_ROWS = 11 };
+
+__device__ inline void x_line(const Consts& K, const double q[5], double o[7]) {
+    double fw[4], cq[3][3], c4[5];
+    weno_weights(q, fw);
+    weno_polys(K, q, cq, c4);
+    weno_at(fw, cq, c4, -0.5, &o[0], &o[2]);
+    weno_at(fw, cq, c4, 0.5, &o[1], &o[3]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) o[4 + a] = weno_val(fw, cq, c4, K.gnode[a]);
+}
+
+__device__ inline void y_line(const Consts& K, const double q[5], bool near, int sh, double o[4]) {
+    if (near) {
+        wall_at(K, sh, q, -0.5, &o[0], &o[2]);
+        wall_at(K, sh, q, 0.5, &o[1], &o[3]);
+    } else {
+        double fw[4], cq[3][3], c4[5];
+        weno_weights(q, fw);
+        weno_polys(K, q, cq, c4);
+        weno_at(fw, cq, c4, -0.5, &o[0], &o[2]);
+        weno_at(fw, cq, c4, 0.5, &o[1], &o[3]);
+    }
+}
+
 // companion evaluation at the three Gauss nodes: value and derivative of the
 // reconstruction of q, and the companion reconstructed with q's weights
 __device__ inline void companion3(const Consts& K, const double q[5], const double comp[5], double d[3][3]) {

@@ -53,6 +53,8 @@ struct Dev {
     double *Wn, *hn, *bn;
     // reconstruction outputs (upwind face data, [face][node][6][nvel])
     double *xf, *yf, *wb, *wt;
+    // first-stage lines of h and b: lines[((cell*LINE_ROWS + row)*nvel + k)*2 + field]
+    double* lines;
     // macro reconstruction: ws[cell][m][side][node][2] (dn, dt per cell width), wv[cell][m][9]
     double *ws, *wv;
     // face fluxes (carry dt, per unit length): x: east face of cell, y: south face of row j (ny+1 rows)
