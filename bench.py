@@ -1,21 +1,30 @@
-"""Throughput of the HighOrderImplicit pseudo-time iteration on one MI355X.
+"""Throughput of the UGKS2D pseudo-time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|shipped]
-                    [--no-cpu] [--copy-calibration]
+                    [--legs hoi,explicit,batched] [--no-cpu] [--copy-calibration]
 
-A step is one pseudo-time iteration of the Couette solver (physical operator:
-time step, WENO5-AO reconstruction, UGKS face and wall fluxes, volume targets,
+Headline (`value`): one HighOrderImplicit pseudo-time iteration of the
+Couette solver at 64x512 cells x 64x64 velocities (physical operator: time
+step, WENO5-AO reconstruction, UGKS face and wall fluxes, volume targets,
 Shakhov collision, residual norms; then the implicit update: PR(2) macro
 Gauss-Seidel, predicted equilibrium, one LU-SGS forward/backward pair and the
 closed-Couette mass rescale), i.e. Solve_UGKS's HighOrderImplicit iteration
-(reference src/mod_Solve.f90:115-165) with the operator of each accepted state
-reused for the next update.  The state starts as the cold-start equilibrium and
-stays resident in HBM; every step runs the full path.
+(reference src/mod_Solve.f90:115-165) with the operator of each accepted
+state reused for the next update.  The state starts as the cold-start
+equilibrium and stays resident in HBM; every step runs the full path.
+`value` is FP64 cell-velocity updates per second (nx*ny*Nu*Nv per
+iteration); `roofline` covers the dominant kernel, timed with HIP events on
+the library's stream over the timed steps; `cpu_baseline` times the C oracle
+on the host cores over a bounded slab of the same workload.
 
-Prints one JSON line.  `value` is FP64 cell-velocity updates per second
-(nx*ny*Nu*Nv per iteration); `roofline` covers the dominant kernel, timed with
-HIP events on the library's stream over the timed steps; `cpu_baseline` times
-the C oracle on the host cores over a bounded slab of the same workload.
+Two more legs go into the same JSON line:
+- "explicit": the low-order Explicit entry on the same grid (one step =
+  TimeStep, EvolutionNative, ApplyExplicitUpdate), the HBM-roofline case,
+  with its own CPU oracle sample;
+- "batched": 64 seeded Couette instances (nx = 8, ny in {32..128}, 32x32
+  velocities, ragged Kn, wall Mach and wall temperature) advanced by one
+  captured hipGraph per iteration after the autotuner picked the launch
+  shapes, against the plain per-instance loop on the same GPU.
 """
 from __future__ import annotations
 
@@ -29,24 +38,27 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic HBM bytes of one iteration per cell-velocity: read h,b and write
-# the streaming+collision rates (4 cv-arrays), read h,b,rates and write the
-# forward corrections (6), read h,b,rates,corrections and write the new h,b
-# (8): 18 FP64 cv-arrays.
+# Algorithmic HBM bytes of one HighOrderImplicit iteration per cell-velocity:
+# read h,b and write the streaming+collision rates (4 cv-arrays), read
+# h,b,rates and write the forward corrections (6), read h,b,rates,corrections
+# and write the new h,b (8): 18 FP64 cv-arrays.
 HOI_BYTES_PER_CV = 18 * 8
+# Algorithmic HBM bytes of one Explicit step per cell-velocity: read h,b and
+# write the new h,b (4 FP64 cv-arrays).
+EXPLICIT_BYTES_PER_CV = 4 * 8
 
 # FP64 operations of the operator kernels per unit of work, from rocprofv3
-# --pmc passes over this workload (ADD + MUL + 2 FMA + TRANS F64 VALU
-# instructions x 64 lanes per launch, divided by the launch's units), and HBM
-# bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB -> bytes; the
+# --pmc passes over the 64x512 x 64^2 workload (ADD + MUL + 2 FMA + TRANS F64
+# VALU instructions x 64 lanes per launch, divided by the launch's units), and
+# HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB -> bytes; the
 # factors make the counters read a 1 GiB device copy as 1 GiB each way).
 # Units: "cv" = cell-velocities nx*ny*Nvel, "fv" = interior face-velocities
 # (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
 KERNEL_COUNTS = {
     # kernel: (flop per unit, unit, HBM bytes per launch); 64x512 cells x 64x64 velocities
-    "k_recon": (16250.1, "cv", 4.9571e10),
-    "k_face_flux": (1827.4, "fv", 8.8191e10),
-    "k_cell": (3065.3, "cv", 1.5597e10),
+    "k_recon": (None, "cv", None),
+    "k_face_flux": (1827.4, "fv", None),
+    "k_cell": (3065.3, "cv", None),
 }
 
 MI355X_HBM_GBPS = 8000.0
@@ -59,7 +71,8 @@ def _args():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["large", "shipped"], default="large")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="hoi,explicit,batched")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -76,6 +89,10 @@ def _case(name):
         desc = ("HighOrderImplicit Couette, 8x64 cells x 32x32 velocities (half-range Gauss-Hermite 32, scale 2), "
                 "Ma=3 diffuse walls, Kn=0.1, Shakhov Pr=2/3, omega=0.81, PR(2) + one LU-SGS pair, mass rescale")
     return case, desc
+
+
+def _threads():
+    return min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
 
 
 def cpu_baseline(case, threads):
@@ -109,6 +126,35 @@ def cpu_baseline(case, threads):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def cpu_baseline_explicit(case, threads):
+    """Oracle Explicit steps on a 16-row slab of the workload (same x and
+    velocity grid), after one untimed step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from ugks2d_high_order_amd import velocity
+    oracle.build()
+    slab = dataclasses.replace(case, ny=16)
+    u, v, w = velocity.table(slab.table, slab.table_scale)
+    o = oracle.OracleSolver(slab, u, v, w, threads=threads)
+    o.explicit_cold_start()
+    st, _, _ = o.explicit_step()
+    if st:
+        raise RuntimeError(f"oracle explicit step failed ({st})")
+    n, t0 = 0, time.perf_counter()
+    while True:
+        st, _, _ = o.explicit_step()
+        if st:
+            raise RuntimeError(f"oracle explicit step failed ({st})")
+        n += 1
+        el = time.perf_counter() - t0
+        if el > 5.0 or n >= 20:
+            break
+    cv = slab.nx * slab.ny * slab.nu * slab.nv * n
+    return {"value": cv / el, "unit": "cell-velocity updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} Explicit steps of a {slab.nx}x{slab.ny}-cell slab with the same {slab.nu}x{slab.nv} "
+                      f"velocity grid, after one untimed step; {el:.1f} s on {threads} OpenMP threads"}
+
+
 def copy_calibration():
     import torch
     n = (1 << 30) // 8
@@ -119,6 +165,88 @@ def copy_calibration():
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}))
+
+
+def _kernel_table(kt):
+    return {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0]) if v[1]}
+
+
+def leg_hoi(U, case, steps, warmup):
+    g = U.UGKS(case)
+    g.cold_start()
+    for _ in range(warmup):
+        acc, _d = g.implicit_step()
+        if not acc:
+            raise RuntimeError("warm-up iteration rejected")
+    g.sync()
+    g.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        acc, _d = g.implicit_step()
+        if not acc:
+            raise RuntimeError("iteration rejected")
+    g.sync()
+    t1 = time.perf_counter()
+    g.profile(False)
+    kt = g.profile_read()
+    g.close()
+    return t1 - t0, kt
+
+
+def leg_explicit(U, case, steps, warmup):
+    g = U.UGKS(case)
+    g.explicit_cold_start()
+    for _ in range(warmup):
+        g.explicit_step()
+    g.sync()
+    g.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.explicit_step()
+    g.sync()
+    t1 = time.perf_counter()
+    g.profile(False)
+    kt = g.profile_read()
+    wg = g.get_tuning(U.UGKS.TUNE_EXPLICIT_WG)
+    g.close()
+    return t1 - t0, kt, wg
+
+
+def leg_batched(U, iterations):
+    from ugks2d_high_order_amd import config
+    cases = config.batched_cases()
+    cv = sum(c.nx * c.ny * c.nu * c.nv for c in cases)
+    B = U.Batch(cases)
+    B.cold_start()
+    acc, st = B.run(1)
+    (wg, tile), table = B.autotune(reps=3)
+    B.run(1)
+    t0 = time.perf_counter()
+    acc, st = B.run(iterations)
+    t1 = time.perf_counter()
+    batched = (t1 - t0) / iterations
+    # the plain loop: the same instances (same shapes), one ugks_implicit_step at a time
+    views = [B.instance(i) for i in range(B.n)]
+    for g in views:
+        g.implicit_step()
+    t0 = time.perf_counter()
+    for g in views:
+        for _ in range(iterations):
+            g.implicit_step()
+    for g in views:
+        g.sync()
+    t1 = time.perf_counter()
+    plain = (t1 - t0) / iterations
+    B.close()
+    return {"instances": len(cases), "cv_per_iteration": cv, "iterations_timed": iterations,
+            "ms_per_iteration_batched": batched * 1e3, "ms_per_iteration_plain_loop": plain * 1e3,
+            "value": cv / batched, "plain_value": cv / plain, "unit": "cell-velocity updates/s",
+            "speedup_vs_plain": plain / batched, "autotuned_shape": {"velocity_workgroup": wg, "sweep_tile": tile},
+            "autotune_ms_per_iteration": {f"{a}x{b}": round(v, 3) for (a, b), v in table.items()},
+            "accepted_all": bool((acc == iterations).all()),
+            "workload": "64 seeded Couette instances (PCG64 seed 20261203): nx=8, ny uniform in {32,64,96,128}, "
+                        "32x32 velocities, Kn log-uniform [0.01,1], wall Mach uniform [0.5,3], wall/initial "
+                        "temperature ratio uniform [0.8,1.2]; one captured hipGraph per iteration"}
 
 
 def main():
@@ -136,63 +264,47 @@ def main():
         copy_calibration()
         return
     import ugks2d_high_order_amd as U
-    from ugks2d_high_order_amd import velocity
+    legs = set(a.legs.split(","))
     case, desc = _case(a.workload)
-    u, v, w = velocity.table(case.table, case.table_scale)
-    g = U.UGKS(case, u, v, w)
-    g.cold_start()
-    for _ in range(a.warmup):
-        acc, _d = g.implicit_step()
-        if not acc:
-            raise RuntimeError("warm-up iteration rejected")
-    g.sync()
-    torch.cuda.synchronize()
-    g.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        acc, _d = g.implicit_step()
-        if not acc:
-            raise RuntimeError("iteration rejected")
-    g.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    g.profile(False)
-    kt = g.profile_read()
-    el = t1 - t0
     cv = case.nx * case.ny * case.nu * case.nv
-    value = cv * a.steps / el
-
-    # dominant kernel: largest summed event time
-    name = max(kt, key=lambda k: kt[k][0])
-    ms, launches = kt[name]
-    avg_s = ms / max(launches, 1) / 1e3
-    flop, unit, traffic = KERNEL_COUNTS.get(name, (None, None, None))
-    units = {"cv": cv, "fv": (case.nx * case.ny + case.nx * (case.ny - 1)) * case.nu * case.nv}.get(unit)
-    if flop is not None and units:
-        ach = flop * units / avg_s / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 valu (no dense contraction, so no MFMA)", "kernel": name,
-                "achieved": ach, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_TFLOPS,
-                "traffic": traffic, "flop_per_unit": flop, "work_unit": unit, "avg_launch_ms": avg_s * 1e3}
-    else:
-        roof = {"bound": "mfma", "pipe": "fp64 valu", "kernel": name, "achieved": None,
-                "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": None, "traffic": traffic,
-                "avg_launch_ms": avg_s * 1e3}
-    kernels = {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0]) if v[1]}
-
-    cpu = None
-    if not a.no_cpu:
-        threads = min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
-        cpu = cpu_baseline(case, threads)
-
-    out = {
-        "metric": "FP64 cell-velocity updates/s", "value": value, "unit": "cell-velocity updates/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "hbm_GBps_algorithmic": HOI_BYTES_PER_CV * value / 1e9,
-        "config": {"workload": desc, "nx": case.nx, "ny": case.ny, "nu": case.nu, "nv": case.nv,
-                   "iterations_timed": a.steps},
-        "roofline": roof, "cpu_baseline": cpu, "kernel_avg_ms": kernels,
-    }
+    out = {"metric": "FP64 cell-velocity updates/s", "value": None, "unit": "cell-velocity updates/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": None, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": desc, "nx": case.nx, "ny": case.ny, "nu": case.nu, "nv": case.nv,
+                      "iterations_timed": a.steps}}
+    threads = _threads()
+    if "hoi" in legs:
+        el, kt = leg_hoi(U, case, a.steps, a.warmup)
+        value = cv * a.steps / el
+        name = max(kt, key=lambda k: kt[k][0])
+        ms, launches = kt[name]
+        avg_s = ms / max(launches, 1) / 1e3
+        flop, unit, traffic = KERNEL_COUNTS.get(name, (None, None, None))
+        units = {"cv": cv, "fv": (case.nx * case.ny + case.nx * (case.ny - 1)) * case.nu * case.nv}.get(unit)
+        ach = flop * units / avg_s / 1e12 if (flop is not None and units) else None
+        out.update(value=value, ms_per_step=el / a.steps * 1e3, hbm_GBps_algorithmic=HOI_BYTES_PER_CV * value / 1e9)
+        out["roofline"] = {"bound": "mfma", "pipe": "fp64 valu (no dense contraction, so no MFMA)", "kernel": name,
+                           "achieved": ach, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                           "frac": None if ach is None else ach / MI355X_FP64_TFLOPS, "traffic": traffic,
+                           "flop_per_unit": flop, "work_unit": unit, "avg_launch_ms": avg_s * 1e3}
+        out["kernel_avg_ms"] = _kernel_table(kt)
+        out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(case, threads)
+    if "explicit" in legs and a.workload == "large":
+        el, kt, wg = leg_explicit(U, case, a.steps, a.warmup)
+        v = cv * a.steps / el
+        ach = EXPLICIT_BYTES_PER_CV * v / 1e9
+        out["explicit"] = {
+            "workload": "low-order Explicit entry (Van Leer, analytic-moment UGKS flux, W2B diffuse walls, "
+                        "trapezoidal Shakhov update) on the same 64x512 x 64x64 grid, freestream start",
+            "steps": a.steps, "ms_per_step": el / a.steps * 1e3, "value": v, "unit": "cell-velocity updates/s",
+            "workgroup": wg,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
+                         "frac": ach / MI355X_HBM_GBPS, "traffic": None,
+                         "basis": "whole step, 32 algorithmic bytes per cell-velocity (read h,b; write h,b)"},
+            "kernel_avg_ms": _kernel_table(kt),
+            "cpu_baseline": None if a.no_cpu else cpu_baseline_explicit(case, threads)}
+    if "batched" in legs:
+        out["batched"] = leg_batched(U, max(3, a.steps // 4))
     print(json.dumps(out), flush=True)
 
 

@@ -32,7 +32,7 @@ def flags():
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(HERE, "..", "include"),
         "-Wno-unused-result",
-    ]
+    ] + [f"-D{d}" for d in os.environ.get("UGKS_DEFINES", "").split()]
 
 
 def _stale() -> bool:

@@ -213,8 +213,12 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     D.wall_ux[0] = p->wall_ux[0]; D.wall_ux[1] = p->wall_ux[1];
     D.wall_tn[0] = p->wall_tn[0]; D.wall_tn[1] = p->wall_tn[1];
     D.macro_sweeps = p->macro_sweeps;
-    D.tile = 16;
-    D.wg_vel = 128;
+    // default launch shapes (ugks_set_tuning keys 1 and 3 change them): a
+    // 256-thread velocity workgroup for the largest velocity grid, and a sweep
+    // tile that gives about one workgroup slot per cell of the longest diagonal
+    D.wg_vel = nvel >= 4096 ? 256 : 128;
+    D.tile = 256 / std::max(8, std::min(32, p->nx));
+    D.tile = D.tile < MIN_TILE ? MIN_TILE : (D.tile > 32 ? 32 : D.tile);
     D.ntiles = nvel / D.tile;
     D.umax = -1e300; D.vmax = -1e300;
     for (int k = 0; k < nvel; ++k) { D.umax = std::max(D.umax, u[k]); D.vmax = std::max(D.vmax, v[k]); }

@@ -167,6 +167,8 @@ __device__ inline void mom_add(double m[4], double w, double u, double v, double
 // small dense solves (uniform across the workgroup)
 // ---------------------------------------------------------------------------
 // Gaussian elimination with partial pivoting; m column-major (m[c*4+r]).
+// Every array index is a compile-time constant (the row interchange is a
+// select over the candidate rows), so the system stays in registers.
 __device__ inline bool solve4(const double* m, const double* rhs, double* x) {
     double a[4][4], b[4];
 #pragma unroll
@@ -175,25 +177,40 @@ __device__ inline bool solve4(const double* m, const double* rhs, double* x) {
         for (int c = 0; c < 4; ++c) a[r][c] = m[c * 4 + r];
         b[r] = rhs[r];
     }
+#pragma unroll
     for (int p = 0; p < 3; ++p) {
         int row = p;
         double best = fabs(a[p][p]);
+#pragma unroll
         for (int r = p + 1; r < 4; ++r)
             if (fabs(a[r][p]) > best) { best = fabs(a[r][p]); row = r; }
-        if (!(fabs(a[row][p]) > 100.0 * UGKS_DBL_MIN) || !isfinite(a[row][p])) return false;
-        if (row != p) {
-            for (int c = 0; c < 4; ++c) { double t = a[p][c]; a[p][c] = a[row][c]; a[row][c] = t; }
-            double t = b[p]; b[p] = b[row]; b[row] = t;
-        }
+        if (!(best > 100.0 * UGKS_DBL_MIN) || !isfinite(best)) return false;
+#pragma unroll
         for (int r = p + 1; r < 4; ++r) {
-            double f = a[r][p] / a[p][p];
+            const bool sw = row == r;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double t = a[p][c];
+                a[p][c] = sw ? a[r][c] : t;
+                a[r][c] = sw ? t : a[r][c];
+            }
+            const double t = b[p];
+            b[p] = sw ? b[r] : t;
+            b[r] = sw ? t : b[r];
+        }
+#pragma unroll
+        for (int r = p + 1; r < 4; ++r) {
+            const double f = a[r][p] / a[p][p];
+#pragma unroll
             for (int c = p; c < 4; ++c) a[r][c] -= f * a[p][c];
             b[r] -= f * b[p];
         }
     }
     if (!(fabs(a[3][3]) > 100.0 * UGKS_DBL_MIN) || !isfinite(a[3][3])) return false;
+#pragma unroll
     for (int r = 3; r >= 0; --r) {
         double s = 0.0;
+#pragma unroll
         for (int c = r + 1; c < 4; ++c) s += a[r][c] * x[c];
         x[r] = (b[r] - s) / a[r][r];
     }

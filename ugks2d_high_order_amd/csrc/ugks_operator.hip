@@ -16,6 +16,17 @@
 
 __constant__ Consts KC;
 
+// Occupancy target (waves per SIMD) of the face and cell closure kernels;
+// 0 leaves the register budget to the compiler.
+#ifndef UGKS_CLOSURE_WAVES
+#define UGKS_CLOSURE_WAVES 0
+#endif
+#if UGKS_CLOSURE_WAVES > 0
+#define UGKS_CLOSURE_ATTR __attribute__((amdgpu_waves_per_eu(UGKS_CLOSURE_WAVES, 8)))
+#else
+#define UGKS_CLOSURE_ATTR
+#endif
+
 // ===========================================================================
 // time step (one workgroup)
 // ===========================================================================
@@ -248,13 +259,13 @@ template <int NT>
 __device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds) {
 #pragma unroll
     for (int a = 0; a < 20; ++a) red[a] = 0.0;
-    const double r0 = th[0], T = th[3];
-    const double pre = r0 / (UGKS_PI * T);
-    const double i0 = 1.0 / r0, iT = 1.0 / T;
+    const double r0 = th[0], T = uni(th[3]), t1 = uni(th[1]), t2 = uni(th[2]);
+    const double pre = uni(r0 / (UGKS_PI * T));
+    const double i0 = uni(1.0 / r0), iT = uni(1.0 / T);
     const int nvel = D.nvel;
     UGKS_VLOOP(k, NT, nvel) {
         const double u = D.u[k], v = D.v[k], w = D.w[k];
-        double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+        double du = u - t1, dv = v - t2, c2 = du * du + dv * dv;
         double q = pre * exp(-c2 * iT);
         double qb = 0.5 * (double)ck * T * q;
         double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
@@ -398,13 +409,15 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
     if (!closure_solve<NT, true>(ck, D, w0, th, J, qh, lds)) return ST_NEWTON;
     double rho, ux, vy, thg;
     if (!s2_prim(ck, w0, &rho, &ux, &vy, &thg)) return ST_NONFINITE;
-    const double lam = 0.5 / thg;
-    const double coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho;
+    ux = uni(ux); vy = uni(vy); rho = uni(rho);
+    uni<4>(th);
+    const double lam = uni(0.5 / thg);
+    const double coeff = uni(0.8 * (1.0 - D.gas.pr) * lam * lam / rho);
     // analytic Maxwellian of w0 (T_native from conserved variables)
     const double kin = 0.5 * (ux * ux + vy * vy);
-    const double tna = 4.0 * (w0[3] / rho - kin) / (double)(ck + 2);
-    const double lama = 1.0 / tna;
-    const double T = th[3], i0 = 1.0 / th[0], iT = 1.0 / th[3];
+    const double tna = uni(4.0 * (w0[3] / rho - kin) / (double)(ck + 2));
+    const double lama = uni(1.0 / tna);
+    const double T = th[3], i0 = uni(1.0 / th[0]), iT = uni(1.0 / th[3]);
     double red[10];
 #pragma unroll
     for (int a = 0; a < 10; ++a) red[a] = 0.0;
@@ -436,6 +449,7 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
     if (!solve4(J, red + 0, tdx) || !solve4(J, red + 4, tdy) || !solve4(J, mnl, tnl) || !solve4(J, mnr, tnr) ||
         !solve4(J, mtg, ttg))
         return ST_SOLVE;
+    uni<4>(tdx); uni<4>(tdy); uni<4>(tnl); uni<4>(tnr); uni<4>(ttg);
     const double q0x = red[8], q0y = red[9];
     // time slope moments
     double wt4[4] = {0, 0, 0, 0};
@@ -458,6 +472,7 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
     block_sum<4>(wt4, lds);
     double wtime[4] = {-wt4[0], -wt4[1], -wt4[2], -wt4[3]}, ttm[4];
     if (!solve4(J, wtime, ttm)) return ST_SOLVE;
+    uni<4>(ttm);
     // variable-tau time integrals
     double w0c[4] = {w0[0], w0[1], w0[2], w0[3]};
     const double tau = tau_of_w(D.gas, w0c);
@@ -468,6 +483,7 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
     mt[0] = dt - mt[3];
     mt[1] = -tau * mt[0] + mt[4];
     mt[2] = 0.5 * dt * dt - tau * mt[0];
+    uni<5>(mt);
     double fw[4] = {0, 0, 0, 0};
     int fin = 1;
     UGKS_VLOOP(k, NT, nvel) {
@@ -513,7 +529,7 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
 }
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_face_flux(const Dev* __restrict__ DS) {
+__global__ void __launch_bounds__(NT) UGKS_CLOSURE_ATTR k_face_flux(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     UGKS_DYN_SHARED(double, qh);
@@ -804,16 +820,19 @@ template <int NT>
 __device__ inline unsigned volume_targets(const Dev& D, const double W[4], const double* wvc, double* node,
                                           double* amh, double* amb, double* lds) {
     const int nvel = D.nvel, ck = D.gas.ck;
-    double nodes[4][9], avg[4] = {0, 0, 0, 0};
-    for (int m = 0; m < 4; ++m)
-        for (int q = 0; q < 9; ++q) nodes[m][q] = wvc[m * 9 + q];
+    double avg[4] = {0, 0, 0, 0};
+#pragma unroll
     for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
         for (int a = 0; a < 3; ++a)
-            for (int m = 0; m < 4; ++m) avg[m] += KC.gw[a] * KC.gw[cc] * nodes[m][a * 3 + cc];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) avg[m] += KC.gw[a] * KC.gw[cc] * wvc[m * 9 + a * 3 + cc];
+#pragma unroll 1
     for (int n = 0; n < 9; ++n) {
         const int cc = n / 3, a = n % 3;
         double wq[4];
-        for (int m = 0; m < 4; ++m) wq[m] = nodes[m][a * 3 + cc] + W[m] - avg[m];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) wq[m] = wvc[m * 9 + a * 3 + cc] + W[m] - avg[m];
         const double wt2 = KC.gw[a] * KC.gw[cc];
         double th[4], J[16];
         if (!closure_solve<NT, false>(ck, D, wq, th, J, nullptr, lds)) return ST_NEWTON;
@@ -878,7 +897,7 @@ __device__ inline void target_kernels(const double* node, int ck, double u, doub
 }
 
 template <int NT>
-__global__ void __launch_bounds__(NT) k_cell(const Dev* __restrict__ DS) {
+__global__ void __launch_bounds__(NT) UGKS_CLOSURE_ATTR k_cell(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     __shared__ double lds[(NT / 16) * 20];
     __shared__ double node[9 * VN_LEN];
