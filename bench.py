@@ -56,10 +56,14 @@ EXPLICIT_BYTES_PER_CV = 4 * 8
 # (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
 KERNEL_COUNTS = {
     # kernel: (flop per unit, unit, HBM bytes per launch); 64x512 cells x 64x64 velocities
-    "k_recon": (None, "cv", None),
-    "k_face_flux": (1827.4, "fv", None),
-    "k_cell": (3065.3, "cv", None),
+    "k_recon": (7706.4, "cv", 1.2513e11),
+    "k_face_flux": (1916.1, "fv", 8.5755e10),
+    "k_cell": (3250.3, "cv", 1.2915e10),
+    "k_recon_lines": (1760.5, "cv", 3.4334e10),
+    "k_predict": (579.0, "cv", 3.5883e10),
 }
+# measured HBM bytes of one Explicit step (both face launches, walls, cells)
+EXPLICIT_TRAFFIC_PER_STEP = 1.6403e10 + 1.9873e10 + 1.2973e10
 
 MI355X_HBM_GBPS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
@@ -147,7 +151,7 @@ def cpu_baseline_explicit(case, threads):
             raise RuntimeError(f"oracle explicit step failed ({st})")
         n += 1
         el = time.perf_counter() - t0
-        if el > 5.0 or n >= 20:
+        if el > 10.0 or n >= 400:
             break
     cv = slab.nx * slab.ny * slab.nu * slab.nv * n
     return {"value": cv / el, "unit": "cell-velocity updates/s", "cores": threads, "kind": "port",
@@ -299,7 +303,7 @@ def main():
             "steps": a.steps, "ms_per_step": el / a.steps * 1e3, "value": v, "unit": "cell-velocity updates/s",
             "workgroup": wg,
             "roofline": {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
-                         "frac": ach / MI355X_HBM_GBPS, "traffic": None,
+                         "frac": ach / MI355X_HBM_GBPS, "traffic": EXPLICIT_TRAFFIC_PER_STEP,
                          "basis": "whole step, 32 algorithmic bytes per cell-velocity (read h,b; write h,b)"},
             "kernel_avg_ms": _kernel_table(kt),
             "cpu_baseline": None if a.no_cpu else cpu_baseline_explicit(case, threads)}
