@@ -107,6 +107,13 @@ int ugks_cn_step(void* handle, int* accepted, double* diag16);
  * balance (4) (the reference's total0 and wall_balance). */
 int ugks_cn_ledger(void* handle, double* out8);
 
+/* Wall macro fluxes (x dt, per unit length) of the last ugks_evolution,
+ * ugks_cn_step or ugks_explicit_step: out[2][nx][4], the bottom wall face of
+ * every column, then the top.  These are the hface(i,1) and hface(i,ny+1)
+ * fluxes PublishWallDiagnostics turns into wall shear and heat flux
+ * (src/mod_S2NativeAdapter.f90:1075-1097). */
+int ugks_wall_fluxes(void* handle, double* out);
+
 /* Low-order Explicit entry (TimeIntegration_method = 'Explicit'): Van Leer
  * slopes, the analytic-moment UGKS face flux with the projected face
  * equilibrium and its Shakhov part, the multiscale diffuse walls and the

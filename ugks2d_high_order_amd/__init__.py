@@ -25,7 +25,7 @@ EXPORTS = [
     "ugks_profile", "ugks_profile_read", "ugks_set_tuning", "ugks_get_tuning", "ugks_cn_step", "ugks_cn_ledger",
     "ugks_explicit_step", "ugks_explicit_cold_start",
     "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
-    "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune",
+    "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.ugks_cn_step.argtypes = [C.c_void_p, C.POINTER(C.c_int), _D]
     L.ugks_cn_ledger.argtypes = [C.c_void_p, _D]
     L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
+    L.ugks_wall_fluxes.argtypes = [C.c_void_p, _D]
     L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.ugks_batch_create.argtypes = [C.c_int, C.POINTER(Params), _D, _D, _D, C.POINTER(C.c_void_p)]
     L.ugks_batch_destroy.argtypes = [C.c_void_p]
@@ -224,6 +225,13 @@ class UGKS:
         dt = np.zeros(1)
         _check(load_library().ugks_explicit_step(self.handle, _ptr(r), _ptr(dt)))
         return r, float(dt[0])
+
+    def wall_fluxes(self):
+        """[2, nx, 4] wall macro fluxes (x dt, per unit length) of the last
+        operator evaluation or step: bottom wall faces, then top."""
+        o = np.zeros((2, self.case.nx, 4))
+        _check(load_library().ugks_wall_fluxes(self.handle, _ptr(o)))
+        return o
 
     def run(self, n: int) -> int:
         done = C.c_int(0)

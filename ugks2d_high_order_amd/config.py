@@ -171,3 +171,12 @@ def from_config_ini(text: str) -> CouetteCase:
     if wall is not None:
         case.wall_ux = wall
     return case
+
+
+def adjoint_case(**kw) -> CouetteCase:
+    """Adjoint workload: 16x128 cells x 32x32 velocities, walls at -U, +U
+    with U at Ma = 2 on the wall sound speed, Kn = 0.1, omega = 0.81."""
+    u = wall_speed(2.0, 2.0)
+    c = CouetteCase(nx=16, ny=128, nu=32, nv=32, table="half32", omega=0.81,
+                    mu_ref=0.1 / math.sqrt(math.pi), wall_ux=(-u, u))
+    return dataclasses.replace(c, **kw)

@@ -511,6 +511,20 @@ extern "C" int ugks_explicit_step(void* handle, double* res4, double* dt) {
     return E_OK;
 }
 
+// Wall macro fluxes of the last operator evaluation or S2CN2 step: out[2][nx][4],
+// bottom then top wall face of each column (x dt, per unit length).
+extern "C" int ugks_wall_fluxes(void* handle, double* out) {
+    Instance* I = (Instance*)handle;
+    if (!I || !out) return fail(E_ARG, "null argument");
+    const Dev& D = I->D;
+    const size_t row = (size_t)4 * D.nx;
+    HIPCHK(hipMemcpyAsync(out, D.yfw, sizeof(double) * row, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipMemcpyAsync(out + row, D.yfw + (size_t)D.ny * row, sizeof(double) * row, hipMemcpyDeviceToHost,
+                          I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
 extern "C" int ugks_run(void* handle, int n, int* done) {
     if (!handle || !done) return fail(E_ARG, "null argument");
     *done = 0;
