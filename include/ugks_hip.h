@@ -114,6 +114,21 @@ int ugks_cn_ledger(void* handle, double* out8);
  * (src/mod_S2NativeAdapter.f90:1075-1097). */
 int ugks_wall_fluxes(void* handle, double* out);
 
+/* Reverse step of one S2CN2 step (the discrete adjoint; no counterpart in
+ * the reference, whose tangent-linear helpers in mod_WENO5AOScalar.f90 and
+ * mod_CorrectedWF2DClosure.f90 are unused).  W, h, b: the state S_k the step
+ * starts from; lamW, lamh, lamb: the adjoint of the step's result S_{k+1};
+ * gwall: host array [2][nx][4], the adjoint of the step's wall fluxes (the
+ * ugks_wall_fluxes layout).  Out: gW, gh, gb, the adjoint of S_k, and on the
+ * host gpar4 = d/d(mu_ref, Pr, bottom wall speed, top wall speed).  All
+ * state-sized arrays are device pointers in the ugks_get_state layout.  The
+ * step is re-run on S_k first (dt is recomputed from S_k); the conservation
+ * ledger of ugks_cn_step is left unchanged.  Calls with the same handle must
+ * not overlap. */
+int ugks_cn_adjoint_device(void* handle, const double* W, const double* h, const double* b,
+                           const double* lamW, const double* lamh, const double* lamb, const double* gwall,
+                           double* gW, double* gh, double* gb, double* gpar4);
+
 /* Low-order Explicit entry (TimeIntegration_method = 'Explicit'): Van Leer
  * slopes, the analytic-moment UGKS face flux with the projected face
  * equilibrium and its Shakhov part, the multiscale diffuse walls and the

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "ugks2d_high_order_amd", "csrc")
 LIB = os.path.join(HERE, "libugks_emu.so")
-SRCS = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_explicit.hip", "ugks_api.hip"]
+SRCS = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_explicit.hip", "ugks_adjoint.hip", "ugks_api.hip"]
 
 
 def build(force=False):

@@ -26,6 +26,7 @@ EXPORTS = [
     "ugks_explicit_step", "ugks_explicit_cold_start",
     "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
     "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
+    "ugks_cn_adjoint_device",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -78,6 +79,7 @@ def load_library(path: str = LIB_PATH):
     L.ugks_cn_ledger.argtypes = [C.c_void_p, _D]
     L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
     L.ugks_wall_fluxes.argtypes = [C.c_void_p, _D]
+    L.ugks_cn_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_D] + [C.c_void_p] * 3 + [_D]
     L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.ugks_batch_create.argtypes = [C.c_int, C.POINTER(Params), _D, _D, _D, C.POINTER(C.c_void_p)]
     L.ugks_batch_destroy.argtypes = [C.c_void_p]
@@ -170,6 +172,14 @@ class UGKS:
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(self.ncell, self.nvel)
         _check(load_library().ugks_set_state(self.handle, _ptr(W), _ptr(h), _ptr(b)))
 
+    def set_state_device(self, W: int, h: int, b: int):
+        """Copy a state from device memory (addresses of FP64 arrays in the
+        ugks_get_state layout)."""
+        _check(load_library().ugks_set_state_device(self.handle, W, h, b))
+
+    def get_state_device(self, W: int, h: int, b: int):
+        _check(load_library().ugks_get_state_device(self.handle, W, h, b))
+
     def state(self):
         W = np.zeros((self.ncell, 4)); h = np.zeros((self.ncell, self.nvel)); b = np.zeros((self.ncell, self.nvel))
         _check(load_library().ugks_get_state(self.handle, _ptr(W), _ptr(h), _ptr(b)))
@@ -232,6 +242,18 @@ class UGKS:
         o = np.zeros((2, self.case.nx, 4))
         _check(load_library().ugks_wall_fluxes(self.handle, _ptr(o)))
         return o
+
+    def cn_adjoint_device(self, W, h, b, lamW, lamh, lamb, gwall, gW, gh, gb):
+        """Reverse of one S2CN2 step from the state (W, h, b) (device
+        addresses, ints): writes the adjoint of the state into (gW, gh, gb)
+        given the adjoint of the step's result (lamW, lamh, lamb) and of its
+        wall fluxes gwall (host array [2, nx, 4]); returns the gradients with
+        respect to (mu_ref, Pr, bottom wall speed, top wall speed)."""
+        g = np.ascontiguousarray(gwall, dtype=np.float64).reshape(2, self.case.nx, 4)
+        out = np.zeros(4)
+        _check(load_library().ugks_cn_adjoint_device(self.handle, W, h, b, lamW, lamh, lamb, _ptr(g), gW, gh, gb,
+                                                      _ptr(out)))
+        return out
 
     def run(self, n: int) -> int:
         done = C.c_int(0)

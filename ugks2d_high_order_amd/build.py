@@ -11,8 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libugks_hip.so")
-SOURCES = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_explicit.hip", "ugks_api.hip"]
-HEADERS = ["ugks_device.h", "ugks_recon.h", "ugks_state.h", "ugks_prof.h"]
+SOURCES = ["ugks_operator.hip", "ugks_implicit.hip", "ugks_explicit.hip", "ugks_adjoint.hip", "ugks_api.hip"]
+HEADERS = ["ugks_device.h", "ugks_recon.h", "ugks_state.h", "ugks_prof.h", "ugks_node.h", "ugks_adjoint.h"]
 FORTRAN = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN, "couette_driver")
 ARCH = os.environ.get("UGKS_OFFLOAD_ARCH", "gfx950")

@@ -1,0 +1,1179 @@
+// Reverse step (discrete adjoint) of the S2CN2 explicit step on gfx950.
+//
+// Given the state S_k = (W, h, b), the adjoint of the proposal S_{k+1}
+// (lamW, lamh, lamb) and the adjoint of the step's wall fluxes, these
+// kernels form the adjoint of S_k and the gradients with respect to mu_ref,
+// Pr and the two wall speeds.  The forward step is first re-run on S_k with
+// its tapes on (per face node, wall face, volume node and cell scalars); the
+// reverse then runs stage by stage in the opposite order of the forward:
+//
+//   k_adj_cn_cell     Crank-Nicolson Shakhov update (advance_step's collision,
+//                     src/mod_S2D1.f90:1302-1435)
+//   k_adj_targets     volume targets of the proposal and of the state
+//                     (build_targets / make_mke, src/mod_S2D1.f90:99-151,1246-1300)
+//   k_adj_wrecon      macro WENO5-AO reconstruction, then k_adj_wgather
+//   k_adj_face        UGKS interior face nodes (ugks_node, src/mod_S2D1.f90:421-521)
+//   k_adj_wall        multiscale diffuse walls (src/mod_S2MultiscaleWall.f90:45-184)
+//   k_adj_recon       limiter and WENO5-AO reconstruction of h and b
+//                     (limit_cell_outputs, cell_faces9; src/mod_S2D1.f90:315-345,
+//                     src/mod_S2D1Reconstruction.f90:171-204,379-578), then k_adj_gather
+//   k_adj_sum         parameter gradients
+//
+// The discrete-equilibrium closures (solve_q_first,
+// src/mod_CorrectedWF2DClosure.f90:330-392) are reversed through the
+// implicit-function theorem: d theta = J^{-1} d w, so g_w = J^{-T} g_theta
+// with the Newton Jacobian at the root, taken from the tape.  All sums are
+// deterministic (fixed order); scattered contributions are staged per user
+// and gathered per owner instead of being added atomically.
+#include "ugks_adjoint.h"
+#include "ugks_prof.h"
+
+__constant__ Consts KA;
+
+extern "C" hipError_t ugks_set_consts_adjoint(const Consts* k) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(KA), k, sizeof(Consts));
+}
+
+#define UGKS_ADJ_LDS(NT, R) __shared__ double lds[((NT) / 16) * (R)]
+
+// tau(w) reverse: g_w += (d tau / d w)^T g_tau, *g_mu += d tau / d mu_ref g_tau
+__device__ inline void tau_adj(const Gas& gas, const double w[4], double g_tau, double gw[4], double* g_mu) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        Tan t[4], mu(gas.mu_ref, i == 4 ? 1.0 : 0.0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) t[a] = Tan(w[a], a == i ? 1.0 : 0.0);
+        const double d = tau_t(gas.ck, gas.omega, t, mu).d * g_tau;
+        if (i < 4) gw[i] += d;
+        else *g_mu += d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Crank-Nicolson Shakhov update of one cell
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_adj_cn_cell(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    UGKS_ADJ_LDS(NT, 8);
+    const int nvel = D.nvel, c = blockIdx.x;
+    if (c >= D.ncell) return;
+    const size_t cb0 = (size_t)c * nvel;
+    double W[4], Wn[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { W[m] = D.W[4 * c + m]; Wn[m] = D.Wn[4 * c + m]; }
+    const double* cs = D.ctape + (size_t)c * CT_LEN;
+    const double kap[4] = {cs[0], cs[1], cs[2], cs[3]};
+    const double rold[2] = {cs[6], cs[7]}, r[2] = {cs[10], cs[11]};
+    const double bo = cs[12], be = cs[13];
+    const double ux = Wn[1] / Wn[0], vy = Wn[2] / Wn[0], uxo = W[1] / W[0], vyo = W[2] / W[0];
+    const double ib = 1.0 / (1.0 + be);
+    const double* lh = D.a_lamh + cb0;
+    const double* lb = D.a_lamb + cb0;
+    // pass 1: beta (through the update), the heat-flux solution r
+    double s1[3] = {0, 0, 0};
+    UGKS_VLOOP(k, NT, nvel) {
+        const size_t o = cb0 + k;
+        const double rh = D.rh[o] + bo * (D.tg[0][o] + D.tg[2][o] * rold[0] + D.tg[4][o] * rold[1] - D.h[o]);
+        const double rb = D.rb[o] + bo * (D.tg[1][o] + D.tg[3][o] * rold[0] + D.tg[5][o] * rold[1] - D.b[o]);
+        const double eqh = D.tgn[0][o] + D.tgn[2][o] * r[0] + D.tgn[4][o] * r[1];
+        const double eqb = D.tgn[1][o] + D.tgn[3][o] * r[0] + D.tgn[5][o] * r[1];
+        const double hn = (rh + be * eqh) * ib, bn = (rb + be * eqb) * ib;
+        s1[0] += lh[k] * (eqh - hn) * ib + lb[k] * (eqb - bn) * ib;
+        const double geh = be * lh[k] * ib, geb = be * lb[k] * ib;
+        s1[1] += geh * D.tgn[2][o] + geb * D.tgn[3][o];
+        s1[2] += geh * D.tgn[4][o] + geb * D.tgn[5][o];
+    }
+    block_sum<3>(s1, lds);
+    // 2x2 system A r = rhs: g_rhs = A^{-T} g_r, g_A = -g_rhs r^T
+    const double a11 = 1.0 + be - be * kap[0], a21 = -be * kap[1], a12 = -be * kap[2], a22 = 1.0 + be - be * kap[3];
+    const double det = a11 * a22 - a12 * a21;
+    const double y0 = (s1[1] * a22 - a21 * s1[2]) / det, y1 = (a11 * s1[2] - a12 * s1[1]) / det;
+    const double ga11 = -y0 * r[0], ga12 = -y0 * r[1], ga21 = -y1 * r[0], ga22 = -y1 * r[1];
+    double g_be = s1[0] + ga11 * (1.0 - kap[0]) - ga21 * kap[1] - ga12 * kap[2] + ga22 * (1.0 - kap[3]);
+    const double gk[4] = {-be * ga11, -be * ga21, -be * ga12, -be * ga22};
+    // pass 2: adjoints of hs, bs, of the proposal's targets, partial old-target adjoints
+    double s2[5] = {0, 0, 0, 0, 0};   // g_rold0, g_rold1, g_bo, g_ux, g_vy (proposal)
+    UGKS_VLOOP(k, NT, nvel) {
+        const size_t o = cb0 + k;
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+        const double rh = D.rh[o] + bo * (D.tg[0][o] + D.tg[2][o] * rold[0] + D.tg[4][o] * rold[1] - D.h[o]);
+        const double rb = D.rb[o] + bo * (D.tg[1][o] + D.tg[3][o] * rold[0] + D.tg[5][o] * rold[1] - D.b[o]);
+        const double geh = be * lh[k] * ib, geb = be * lb[k] * ib;
+        const double Py = 0.5 * w * (y0 * cx + y1 * cy);
+        const double g_rh = lh[k] * ib + Py * c2, g_rb = lb[k] * ib + Py;
+        D.a_hs[o] = g_rh;
+        D.a_bs[o] = g_rb;
+        const double Pkx = 0.5 * w * (gk[0] * cx + gk[1] * cy), Pky = 0.5 * w * (gk[2] * cx + gk[3] * cy);
+        D.a_tgn[0][o] = geh - Py * c2;
+        D.a_tgn[1][o] = geb - Py;
+        D.a_tgn[2][o] = geh * r[0] + Pkx * c2;
+        D.a_tgn[3][o] = geb * r[0] + Pkx;
+        D.a_tgn[4][o] = geh * r[1] + Pky * c2;
+        D.a_tgn[5][o] = geb * r[1] + Pky;
+        const double kxh = D.tg[2][o], kxb = D.tg[3][o], kyh = D.tg[4][o], kyb = D.tg[5][o];
+        s2[0] += bo * (kxh * g_rh + kxb * g_rb);
+        s2[1] += bo * (kyh * g_rh + kyb * g_rb);
+        s2[2] += (D.tg[0][o] + kxh * rold[0] + kyh * rold[1] - D.h[o]) * g_rh +
+                 (D.tg[1][o] + kxb * rold[0] + kyb * rold[1] - D.b[o]) * g_rb;
+        // bulk-velocity dependence of the three heat fluxes at the proposal
+        const double terms[3][4] = {{y0, y1, rh, rb},
+                                    {-y0, -y1, D.tgn[0][o], D.tgn[1][o]},
+                                    {gk[0], gk[1], D.tgn[2][o], D.tgn[3][o]}};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double g0 = terms[q][0], g1 = terms[q][1], fh = terms[q][2], fb = terms[q][3];
+            const double gc = g0 * cx + g1 * cy, e = c2 * fh + fb;
+            s2[3] += 0.5 * w * (-g0 * e - gc * 2.0 * cx * fh);
+            s2[4] += 0.5 * w * (-g1 * e - gc * 2.0 * cy * fh);
+        }
+        {
+            const double g0 = gk[2], g1 = gk[3], fh = D.tgn[4][o], fb = D.tgn[5][o];
+            const double gc = g0 * cx + g1 * cy, e = c2 * fh + fb;
+            s2[3] += 0.5 * w * (-g0 * e - gc * 2.0 * cx * fh);
+            s2[4] += 0.5 * w * (-g1 * e - gc * 2.0 * cy * fh);
+        }
+    }
+    block_sum<5>(s2, lds);
+    const double g_rold[2] = {s2[0], s2[1]};
+    // pass 3: old-target adjoints, direct adjoints of h and b
+    double s3[2] = {0, 0};
+    UGKS_VLOOP(k, NT, nvel) {
+        const size_t o = cb0 + k;
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        const double cx = u - uxo, cy = v - vyo, c2 = cx * cx + cy * cy;
+        const double g_rh = D.a_hs[o], g_rb = D.a_bs[o];
+        const double P = 0.5 * w * (g_rold[0] * cx + g_rold[1] * cy);
+        const double dh = D.h[o] - D.tg[0][o], db = D.b[o] - D.tg[1][o];
+        D.a_tg[0][o] = bo * g_rh - P * c2;
+        D.a_tg[1][o] = bo * g_rb - P;
+        D.a_tg[2][o] = bo * rold[0] * g_rh;
+        D.a_tg[3][o] = bo * rold[0] * g_rb;
+        D.a_tg[4][o] = bo * rold[1] * g_rh;
+        D.a_tg[5][o] = bo * rold[1] * g_rb;
+        D.a_h[o] = -bo * g_rh + P * c2;
+        D.a_b[o] = -bo * g_rb + P;
+        const double gc = g_rold[0] * cx + g_rold[1] * cy, e = c2 * dh + db;
+        s3[0] += 0.5 * w * (-g_rold[0] * e - gc * 2.0 * cx * dh);
+        s3[1] += 0.5 * w * (-g_rold[1] * e - gc * 2.0 * cy * dh);
+    }
+    block_sum<2>(s3, lds);
+    if (threadIdx.x == 0) {
+        double gWn[4] = {0, 0, 0, 0}, gW[4] = {0, 0, 0, 0}, g_mu = 0.0;
+        const double tn = tau_of_w(D.gas, Wn), to = tau_of_w(D.gas, W);
+        tau_adj(D.gas, Wn, -g_be * be / tn, gWn, &g_mu);
+        tau_adj(D.gas, W, -s2[2] * bo / to, gW, &g_mu);
+        gWn[1] += s2[3] / Wn[0];
+        gWn[2] += s2[4] / Wn[0];
+        gWn[0] -= (s2[3] * ux + s2[4] * vy) / Wn[0];
+        gW[1] += s3[0] / W[0];
+        gW[2] += s3[1] / W[0];
+        gW[0] -= (s3[0] * uxo + s3[1] * vyo) / W[0];
+        for (int m = 0; m < 4; ++m) {
+            D.a_Wn[4 * c + m] = D.a_lamW[4 * c + m] + gWn[m];
+            D.a_W[4 * c + m] = gW[m];
+        }
+        double* gp = D.a_par + 4 * (size_t)c;
+        gp[0] = g_mu; gp[1] = 0.0; gp[2] = 0.0; gp[3] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// volume targets (9 Gauss nodes of the macro WENO tensor) of one cell
+// ---------------------------------------------------------------------------
+// set 0: targets of the state (adjoint a_tg, W), set 1: of the proposal (a_tgn, Wn)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_adj_targets(const Dev* __restrict__ DS, int set) {
+    const Dev& D = DS[blockIdx.z];
+    UGKS_ADJ_LDS(NT, 8);
+    __shared__ double gwq[9][4];
+    const int nvel = D.nvel, ck = D.gas.ck, c = blockIdx.x;
+    if (c >= D.ncell) return;
+    const size_t cb0 = (size_t)c * nvel;
+    double* const* G = set ? D.a_tgn : D.a_tg;
+    const double* gmh = G[0] + cb0;
+    const double* gmb = G[1] + cb0;
+    const double* gkxh = G[2] + cb0;
+    const double* gkxb = G[3] + cb0;
+    const double* gkyh = G[4] + cb0;
+    const double* gkyb = G[5] + cb0;
+    const double* tape = D.ttape + (size_t)(c * 2 + set) * 9 * TT_LEN;
+    double g_pr = 0.0;
+#pragma unroll 1
+    for (int n = 0; n < 9; ++n) {
+        const int cc = n / 3, a = n % 3;
+        const double wt2 = KA.gw[a] * KA.gw[cc];
+        const double* tp = tape + n * TT_LEN;
+        double th[4], J[16], wq[4], tdx[4], tdy[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { th[q] = tp[q]; wq[q] = tp[20 + q]; tdx[q] = tp[24 + q]; tdy[q] = tp[28 + q]; }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) J[q] = tp[4 + q];
+        double rho, ux, vy, thg;
+        s2_prim(ck, wq, &rho, &ux, &vy, &thg);
+        const double lam = 0.5 / thg, coeff = 0.8 * (1.0 - D.gas.pr) * lam * lam / rho, T = th[3];
+        const double pre = th[0] / (UGKS_PI * T), iT = 1.0 / T, i0 = 1.0 / th[0];
+        // R1: the projection directions
+        double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k];
+            const double q = q_of(th, pre, iT, u, v);
+            double qth[4], qtb[4];
+            qtheta(ck, th, i0, iT, u, v, q, qth, qtb);
+            const double a1 = wt2 * gkxh[k], a2 = wt2 * gkxb[k], a3 = wt2 * gkyh[k], a4 = wt2 * gkyb[k];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                S[q4] += qth[q4] * a1 + qtb[q4] * a2;
+                S[4 + q4] += qth[q4] * a3 + qtb[q4] * a4;
+            }
+        }
+        block_sum<8>(S, lds);
+        double gtx[4], gty[4], gmzx[4], gmzy[4], gJ[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { gtx[q] = -S[q]; gty[q] = -S[4 + q]; }
+        solve4T(J, gtx, gmzx);
+        solve4T(J, gty, gmzy);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) gJ[q] = 0.0;
+        outer_sub(gJ, gmzx, tdx);
+        outer_sub(gJ, gmzy, tdy);
+        // R2: Shakhov kernel scalars and the closure theta
+        double R[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // g_coeff, g_lam, g_ux, g_vy, g_theta(4)
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double q = q_of(th, pre, iT, u, v), qb = 0.5 * (double)ck * T * q;
+            double qth[4], qtb[4];
+            qtheta(ck, th, i0, iT, u, v, q, qth, qtb);
+            const double a1 = wt2 * gkxh[k], a2 = wt2 * gkxb[k], a3 = wt2 * gkyh[k], a4 = wt2 * gkyb[k];
+            const double zxh = a1 + mom_adj_h(gmzx, w, u, v), zxb = a2 + 0.5 * w * gmzx[3];
+            const double zyh = a3 + mom_adj_h(gmzy, w, u, v), zyb = a4 + 0.5 * w * gmzy[3];
+            const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            const double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            R[0] += cx * (shh * q * zxh + shb * qb * zxb) + cy * (shh * q * zyh + shb * qb * zyb);
+            const double gsh = coeff * (cx * (q * zxh + qb * zxb) + cy * (q * zyh + qb * zyb));
+            R[1] += 2.0 * c2 * gsh;
+            const double gc2 = 2.0 * lam * gsh;
+            R[2] -= coeff * (shh * q * zxh + shb * qb * zxb) + 2.0 * cx * gc2;
+            R[3] -= coeff * (shh * q * zyh + shb * qb * zyb) + 2.0 * cy * gc2;
+            const double g_q = wt2 * gmh[k] + coeff * shh * (cx * zxh + cy * zyh);
+            const double g_qb = wt2 * gmb[k] + coeff * shb * (cx * zxb + cy * zyb);
+            double g_qt[4], g_qbt[4];
+            const double phi[4] = {w, w * u, w * v, 0.5 * w * (u * u + v * v)};
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                double fold = 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) fold += gJ[q4 * 4 + rr] * phi[rr];
+                g_qt[q4] = -tdx[q4] * a1 - tdy[q4] * a3 + fold;
+                g_qbt[q4] = -tdx[q4] * a2 - tdy[q4] * a4 + 0.5 * w * gJ[q4 * 4 + 3];
+            }
+            qtheta_adj(ck, th, i0, iT, u, v, q, g_q, g_qb, g_qt, g_qbt, R + 4);
+        }
+        block_sum<8>(R, lds);
+        double gw[4];
+        solve4T(J, R + 4, gw);
+        // coeff = 0.8 (1 - Pr) lam^2 / rho, lam = 1/(2 thg), thg = p/rho
+        const double g_coeff = R[0];
+        double g_lam = R[1] + 2.0 * coeff / lam * g_coeff, g_ux = R[2], g_vy = R[3];
+        double g_rho = -coeff / rho * g_coeff;
+        g_pr += -0.8 * lam * lam / rho * g_coeff;
+        const double g_thg = -2.0 * lam * lam * g_lam;
+        const double kp = 2.0 / (double)(ck + 2);
+        gw[3] += g_thg * kp / rho;
+        g_rho += -g_thg * kp * wq[3] / (rho * rho);
+        g_ux += -g_thg * kp * ux;
+        g_vy += -g_thg * kp * vy;
+        gw[1] += g_ux / rho;
+        gw[2] += g_vy / rho;
+        gw[0] += g_rho - (g_ux * ux + g_vy * vy) / rho;
+        if (threadIdx.x == 0)
+            for (int m = 0; m < 4; ++m) gwq[n][m] = gw[m];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // wq = vol + W - avg, avg = sum_n wt2_n vol_n
+        double* Wacc = (set ? D.a_Wn : D.a_W) + 4 * (size_t)c;
+        double* gv = D.a_wv + (size_t)c * 36;
+        for (int m = 0; m < 4; ++m) {
+            double tot = 0.0;
+            for (int n = 0; n < 9; ++n) tot += gwq[n][m];
+            Wacc[m] += tot;
+            for (int n = 0; n < 9; ++n) {
+                const int cc = n / 3, a = n % 3;
+                gv[m * 9 + a * 3 + cc] = gwq[n][m] - KA.gw[a] * KA.gw[cc] * tot;
+            }
+        }
+        D.a_par[4 * (size_t)c + 1] += g_pr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// macro reconstruction of W (or Wn) and its gather
+// ---------------------------------------------------------------------------
+// per (cell, m): the adjoints of the side slopes (a_ws, state only) and of the
+// volume nodes (a_wv) onto the 5x5 block of the cell's stencil, a_blkw[cell][m][25]
+__global__ void __launch_bounds__(256) k_adj_wrecon(const Dev* __restrict__ DS, int proposal) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * D.nx || (int)blockIdx.y >= D.ny) return;
+    const int j = blockIdx.y, i = t >> 2, m = t & 3, c = j * D.nx + i;
+    const double* Wsrc = proposal ? D.Wn : D.W;
+    int rb, sh;
+    bool near;
+    stencil_rows(j, D.ny, &rb, &near, &sh);
+    double blk[5][5];
+    for (int r = 0; r < 5; ++r)
+        for (int s = 0; s < 5; ++s) {
+            int col = i + s - 2;
+            col = col < 0 ? col + D.nx : (col >= D.nx ? col - D.nx : col);
+            blk[r][s] = Wsrc[4 * ((rb + r) * D.nx + col) + m];
+        }
+    FirstLayer F, gL;
+    first_layer(KA, blk, near, sh, F);
+    zero_layer(gL);
+    if (!proposal) {
+        const double* gs = D.a_ws + (size_t)(c * 4 + m) * 24;
+        for (int s = 0; s < 4; ++s) {
+            double g[3][3];
+            for (int a = 0; a < 3; ++a) { g[a][0] = 0.0; g[a][1] = gs[(s * 3 + a) * 2]; g[a][2] = gs[(s * 3 + a) * 2 + 1]; }
+            side_adj(KA, F, near, sh, s, g, gL);
+        }
+    }
+    double gvol[3][3];
+    const double* gv = D.a_wv + (size_t)(c * 4 + m) * 9;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) gvol[a][b] = gv[a * 3 + b];
+    vol_adj(KA, F, near, sh, gvol, gL);
+    double gblk[5][5];
+    for (int r = 0; r < 5; ++r)
+        for (int s = 0; s < 5; ++s) gblk[r][s] = 0.0;
+    first_layer_adj(KA, blk, near, sh, gL, gblk);
+    double* o = D.a_blkw + (size_t)(c * 4 + m) * 25;
+    for (int r = 0; r < 5; ++r)
+        for (int s = 0; s < 5; ++s) o[r * 5 + s] = gblk[r][s];
+}
+
+// sum of the staged block adjoints that land on cell (j, i); fixed order
+__device__ inline double gather_block(const double* blk, size_t stride, size_t lane, int nx, int ny, int i, int j) {
+    double s = 0.0;
+    const int j0 = j - 4 < 0 ? 0 : j - 4, j1 = j + 4 > ny - 1 ? ny - 1 : j + 4;
+    for (int jj = j0; jj <= j1; ++jj) {
+        int rb, sh;
+        bool near;
+        stencil_rows(jj, ny, &rb, &near, &sh);
+        const int r = j - rb;
+        if (r < 0 || r > 4) continue;
+        for (int q = 0; q < 5; ++q) {
+            int ii = i - q + 2;
+            ii = ((ii % nx) + nx) % nx;
+            s += blk[((size_t)(jj * nx + ii) * 25 + r * 5 + q) * stride + lane];
+        }
+    }
+    return s;
+}
+
+// proposal: a_Wn += gathered; state: a_Wout = a_W + a_Wn + wall + gathered
+__global__ void __launch_bounds__(256) k_adj_wgather(const Dev* __restrict__ DS, int proposal) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * D.ncell) return;
+    const int c = t >> 2, m = t & 3, i = c % D.nx, j = c / D.nx;
+    double s = 0.0;
+    const int j0 = j - 4 < 0 ? 0 : j - 4, j1 = j + 4 > D.ny - 1 ? D.ny - 1 : j + 4;
+    for (int jj = j0; jj <= j1; ++jj) {
+        int rb, sh;
+        bool near;
+        stencil_rows(jj, D.ny, &rb, &near, &sh);
+        const int r = j - rb;
+        if (r < 0 || r > 4) continue;
+        for (int q = 0; q < 5; ++q) {
+            int ii = i - q + 2;
+            ii = ((ii % D.nx) + D.nx) % D.nx;
+            s += D.a_blkw[(size_t)((jj * D.nx + ii) * 4 + m) * 25 + r * 5 + q];
+        }
+    }
+    if (proposal) {
+        D.a_Wn[t] += s;
+    } else {
+        double v = D.a_W[t] + D.a_Wn[t] + s;
+        if (j == 0) v += D.a_Wwall[4 * i + m];
+        if (j == D.ny - 1) v += D.a_Wwall[4 * (D.nx + i) + m];
+        D.a_Wout[t] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-node scalar chains: values and tangent derivatives, one thread per
+// (node, direction); the face and wall kernels contract them with the
+// adjoints of the chain outputs
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adj_chains(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = D.nx, ck = D.gas.ck;
+    const int nfaces = D.ncell + nx * (D.ny - 1);
+    const int nf = nfaces * 3 * 7, nw = 2 * nx * 3 * 14;
+    const double dt = D.scal[SC_DT];
+    if (t < nf) {
+        const int node = t / 7, dir = t % 7 - 1;   // dir -1: values; 0-3 w0, 4 Pr, 5 mu_ref
+        const double* tp = D.ftape + (size_t)node * FT_LEN;
+        Tan w0[4], out[FC_N];
+        for (int q = 0; q < 4; ++q) w0[q] = Tan(tp[q], q == dir ? 1.0 : 0.0);
+        const Tan pr(D.gas.pr, dir == 4 ? 1.0 : 0.0), mu(D.gas.mu_ref, dir == 5 ? 1.0 : 0.0);
+        face_chain<Tan>(ck, D.gas.omega, dt, w0, pr, mu, out);
+        double* o = D.a_fjac + (size_t)t * FJ_W;
+        for (int q = 0; q < FC_N; ++q) o[q] = dir < 0 ? out[q].v : out[q].d;
+    } else if (t < nf + nw) {
+        const int u = t - nf;
+        const int node = u / 14, dir = u % 14 - 1;   // dir -1: values; else a wall-chain input
+        const int wf = node / 3, a = node % 3;
+        const int top = wf >= nx ? 1 : 0, i = wf % nx;
+        const int c = top ? (D.ny - 1) * nx + i : i;
+        const double* tape = D.wtape + (size_t)wf * WT_LEN;
+        double in[WI_N];
+        for (int m = 0; m < 4; ++m) in[WI_WC + m] = D.W[4 * c + m];
+        in[WI_UX] = D.wall_ux[top]; in[WI_MU] = D.gas.mu_ref; in[WI_PR] = D.gas.pr;
+        for (int q = 0; q < 6; ++q) in[WI_RED + q] = tape[20 + 8 * a + q];
+        Tan tin[WI_N], out[WC_N];
+        for (int q = 0; q < WI_N; ++q) tin[q] = Tan(in[q], q == dir ? 1.0 : 0.0);
+        wall_chain<Tan>(ck, D.gas.omega, dt, D.dy, top ? 1.0 : -1.0, 0.5 * D.wall_tn[top], tin, out);
+        double* o = D.a_wjac + (size_t)u * WJ_W;
+        for (int q = 0; q < WC_N; ++q) o[q] = dir < 0 ? out[q].v : out[q].d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UGKS interior face: three Gauss nodes, one workgroup per face
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_adj_face(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    UGKS_ADJ_LDS(NT, 20);
+    const int nx = D.nx, nvel = D.nvel, ck = D.gas.ck;
+    const size_t nv = (size_t)nvel;
+    const int f = blockIdx.x, nfx = D.ncell;
+    if (f >= nfx + nx * (D.ny - 1)) return;
+    int axis, cl, cr;
+    const double* rec;
+    double* grec;
+    if (f < nfx) {
+        axis = 1;
+        const int j = f / nx, i = f % nx;
+        cl = f;
+        cr = j * nx + (i == nx - 1 ? 0 : i + 1);
+        rec = D.xf + (size_t)f * REC_ROWS * nv;
+        grec = D.a_xf + (size_t)f * REC_ROWS * nv;
+    } else {
+        axis = 2;
+        const int g = f - nfx;
+        cl = g;
+        cr = g + nx;
+        rec = D.yf + (size_t)g * REC_ROWS * nv;
+        grec = D.a_yf + (size_t)g * REC_ROWS * nv;
+    }
+    const double area = D.dx * D.dy, sfac = (axis == 1 ? D.dy : D.dx) / area;
+    const double dt = D.scal[SC_DT];
+    const double ln = axis == 1 ? D.dx : D.dy, lt = axis == 1 ? D.dy : D.dx;
+    const int sl = axis == 1 ? 1 : 3, sr = axis == 1 ? 0 : 2;
+    // adjoint of the face's macro flux (the divergence reversed)
+    double gfw[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) gfw[m] = uni(sfac * (D.a_Wn[4 * cr + m] - D.a_Wn[4 * cl + m]));
+    const double* ghr = D.a_hs + (size_t)cr * nv;
+    const double* ghl = D.a_hs + (size_t)cl * nv;
+    const double* gbr = D.a_bs + (size_t)cr * nv;
+    const double* gbl = D.a_bs + (size_t)cl * nv;
+    double g_pr = 0.0, g_mu = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        const double gwa = KA.gw[a];
+        const double* tp = D.ftape + (size_t)(f * 3 + a) * FT_LEN;
+        const double* q6 = rec + (size_t)(a * 6) * nv;
+        double* g6 = grec + (size_t)(a * 6) * nv;
+        double w0[4], th[4], J[16], mzx[4], mzy[4], wt4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            w0[q] = tp[q]; th[q] = tp[4 + q]; mzx[q] = tp[24 + q]; mzy[q] = tp[28 + q]; wt4[q] = tp[34 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) J[q] = tp[8 + q];
+        const double q0x = tp[32], q0y = tp[33];
+        const double* jac = D.a_fjac + (size_t)(f * 3 + a) * 7 * FJ_W;
+        double sc[FC_N];
+#pragma unroll
+        for (int q = 0; q < FC_N; ++q) sc[q] = jac[q];
+        const double rho = sc[FC_RHO], ux = sc[FC_UX], vy = sc[FC_VY], lam = sc[FC_LAM], coeff = sc[FC_COEFF];
+        const double tna = sc[FC_TNA], lama = sc[FC_LAMA];
+        const double mt[5] = {sc[FC_M0], sc[FC_M0 + 1], sc[FC_M0 + 2], sc[FC_M0 + 3], sc[FC_M0 + 4]};
+        const double T = th[3], i0 = 1.0 / th[0], iT = 1.0 / T, pre = th[0] / (UGKS_PI * T);
+        double mnl[4], mnr[4], mtg[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double* pl = D.ws + (size_t)(cl * 4 + m) * 24 + (sl * 3 + a) * 2;
+            const double* pr = D.ws + (size_t)(cr * 4 + m) * 24 + (sr * 3 + a) * 2;
+            mnl[m] = pl[0] / ln;
+            mnr[m] = pr[0] / ln;
+            mtg[m] = 0.5 * (pl[1] + pr[1]) / lt;
+        }
+        double tdx[4], tdy[4], tnl[4], tnr[4], ttg[4], ttm[4];
+        const double wtime[4] = {-wt4[0], -wt4[1], -wt4[2], -wt4[3]};
+        solve4(J, mzx, tdx); solve4(J, mzy, tdy); solve4(J, mnl, tnl); solve4(J, mnr, tnr); solve4(J, mtg, ttg);
+        solve4(J, wtime, ttm);
+        // ---- R1: time-integral, heat and projection adjoints
+        double R1[11];
+#pragma unroll
+        for (int q = 0; q < 11; ++q) R1[q] = 0.0;
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
+            const double Gh = gwa * (sfac * (ghr[k] - ghl[k]) + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sfac * (gbr[k] - gbl[k]) + 0.5 * w * gfw[3]);
+            const double Ah = vn * Gh, Ab = vn * Gb;
+            const double mh = q_of(th, pre, iT, u, v), mb = 0.5 * (double)ck * T * mh;
+            double qth[4], qtb[4];
+            qtheta(ck, th, i0, iT, u, v, mh, qth, qtb);
+            const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            const double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            double kxh = coeff * cx * shh * mh, kxb = coeff * cx * shb * mb;
+            double kyh = coeff * cy * shh * mh, kyb = coeff * cy * shb * mb;
+            double nlh = 0, nlb = 0, nrh = 0, nrb = 0, tgh = 0, tgb = 0, tmh = 0, tmb = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                kxh -= qth[q] * tdx[q]; kxb -= qtb[q] * tdx[q];
+                kyh -= qth[q] * tdy[q]; kyb -= qtb[q] * tdy[q];
+                nlh += qth[q] * tnl[q]; nlb += qtb[q] * tnl[q];
+                nrh += qth[q] * tnr[q]; nrb += qtb[q] * tnr[q];
+                tgh += qth[q] * ttg[q]; tgb += qtb[q] * ttg[q];
+                tmh += qth[q] * ttm[q]; tmb += qtb[q] * ttm[q];
+            }
+            const double s0h = mh + kxh * q0x + kyh * q0y, s0b = mb + kxb * q0x + kyb * q0y;
+            const bool up = vn >= 0.0;
+            const double slh = vn * (up ? nlh : nrh) + vt * tgh, slb = vn * (up ? nlb : nrb) + vt * tgb;
+            const double f0h = q6[k], fnh = q6[nv + k], fth = q6[2 * nv + k];
+            const double f0b = q6[3 * nv + k], fnb = q6[4 * nv + k], ftb = q6[5 * nv + k];
+            R1[0] += Ah * s0h + Ab * s0b;
+            R1[1] += Ah * slh + Ab * slb;
+            R1[2] += Ah * tmh + Ab * tmb;
+            R1[3] += Ah * f0h + Ab * f0b;
+            R1[4] -= Ah * (vn * fnh + vt * fth) + Ab * (vn * fnb + vt * ftb);
+            R1[5] += mt[0] * (Ah * kxh + Ab * kxb);
+            R1[6] += mt[0] * (Ah * kyh + Ab * kyb);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) R1[7 + q] += qth[q] * Ah + qtb[q] * Ab;
+        }
+        block_sum<11>(R1, lds);
+        const double g_q0x = R1[5], g_q0y = R1[6];
+        double gtm[4], gtx[4], gty[4], gwt[4], gmzx[4], gmzy[4], gJ[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            gtm[q] = mt[2] * R1[7 + q];
+            gtx[q] = -q0x * mt[0] * R1[7 + q];
+            gty[q] = -q0y * mt[0] * R1[7 + q];
+        }
+        solve4T(J, gtm, gwt); solve4T(J, gtx, gmzx); solve4T(J, gty, gmzy);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) gJ[q] = 0.0;
+        outer_sub(gJ, gwt, ttm); outer_sub(gJ, gmzx, tdx); outer_sub(gJ, gmzy, tdy);
+        // ---- R2: macro-slope projections, Shakhov kernel scalars, the heat flux q0
+        double R2[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) R2[q] = 0.0;   // T_nl(4) T_nr(4) T_tg(4) coeff lam ux vy tna rho lama
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
+            const double Gh = gwa * (sfac * (ghr[k] - ghl[k]) + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sfac * (gbr[k] - gbl[k]) + 0.5 * w * gfw[3]);
+            const double Ah = vn * Gh, Ab = vn * Gb;
+            const double mh = q_of(th, pre, iT, u, v), mb = 0.5 * (double)ck * T * mh;
+            double qth[4], qtb[4];
+            qtheta(ck, th, i0, iT, u, v, mh, qth, qtb);
+            const bool up = vn >= 0.0;
+            const double gslh = mt[1] * Ah - mom_adj_h(gwt, w, u, v), gslb = mt[1] * Ab - 0.5 * w * gwt[3];
+            const double gnlh = up ? vn * gslh : 0.0, gnlb = up ? vn * gslb : 0.0;
+            const double gnrh = up ? 0.0 : vn * gslh, gnrb = up ? 0.0 : vn * gslb;
+            const double gtgh = vt * gslh, gtgb = vt * gslb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                R2[q] += qth[q] * gnlh + qtb[q] * gnlb;
+                R2[4 + q] += qth[q] * gnrh + qtb[q] * gnrb;
+                R2[8 + q] += qth[q] * gtgh + qtb[q] * gtgb;
+            }
+            const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            const double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            const double zxh = q0x * mt[0] * Ah + mom_adj_h(gmzx, w, u, v), zxb = q0x * mt[0] * Ab + 0.5 * w * gmzx[3];
+            const double zyh = q0y * mt[0] * Ah + mom_adj_h(gmzy, w, u, v), zyb = q0y * mt[0] * Ab + 0.5 * w * gmzy[3];
+            R2[12] += cx * (shh * mh * zxh + shb * mb * zxb) + cy * (shh * mh * zyh + shb * mb * zyb);
+            const double gsh = coeff * (cx * (mh * zxh + mb * zxb) + cy * (mh * zyh + mb * zyb));
+            R2[13] += 2.0 * c2 * gsh;
+            const double gc2 = 2.0 * lam * gsh;
+            R2[14] -= coeff * (shh * mh * zxh + shb * mb * zxb) + 2.0 * cx * gc2;
+            R2[15] -= coeff * (shh * mh * zyh + shb * mb * zyb) + 2.0 * cy * gc2;
+            // q0 = heat(w0, f0 - ea)
+            const double eah = rho * lama / UGKS_PI * exp(-lama * c2), eab = 0.5 * (double)ck * tna * eah;
+            const double dh = q6[k] - eah, db = q6[3 * nv + k] - eab;
+            const double P = 0.5 * w * (g_q0x * cx + g_q0y * cy), e = c2 * dh + db;
+            R2[14] += 0.5 * w * (-g_q0x * e) - P * 2.0 * cx * dh;
+            R2[15] += 0.5 * w * (-g_q0y * e) - P * 2.0 * cy * dh;
+            const double gea = -P * (c2 + 0.5 * (double)ck * tna);
+            R2[16] += -P * 0.5 * (double)ck * eah;
+            R2[17] += gea * eah / rho;
+            R2[18] += gea * eah * (1.0 / lama - c2);
+            R2[14] += gea * eah * lama * 2.0 * cx;
+            R2[15] += gea * eah * lama * 2.0 * cy;
+        }
+        block_sum<19>(R2, lds);
+        double gml[4], gmr[4], gmt[4];
+        solve4T(J, R2 + 0, gml); solve4T(J, R2 + 4, gmr); solve4T(J, R2 + 8, gmt);
+        outer_sub(gJ, gml, tnl); outer_sub(gJ, gmr, tnr); outer_sub(gJ, gmt, ttg);
+        // ---- R3: the closure theta
+        double gth[4] = {0, 0, 0, 0};
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
+            const double Gh = gwa * (sfac * (ghr[k] - ghl[k]) + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sfac * (gbr[k] - gbl[k]) + 0.5 * w * gfw[3]);
+            const double Ah = vn * Gh, Ab = vn * Gb;
+            const double mh = q_of(th, pre, iT, u, v);
+            const bool up = vn >= 0.0;
+            const double gslh = mt[1] * Ah - mom_adj_h(gwt, w, u, v), gslb = mt[1] * Ab - 0.5 * w * gwt[3];
+            const double gnlh = up ? vn * gslh : 0.0, gnlb = up ? vn * gslb : 0.0;
+            const double gnrh = up ? 0.0 : vn * gslh, gnrb = up ? 0.0 : vn * gslb;
+            const double gtgh = vt * gslh, gtgb = vt * gslb;
+            const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            const double shh = 2.0 * lam * c2 - 4.0, shb = shh + 2.0 / (double)ck;
+            const double gkxh = q0x * mt[0] * Ah, gkxb = q0x * mt[0] * Ab;
+            const double gkyh = q0y * mt[0] * Ah, gkyb = q0y * mt[0] * Ab;
+            const double zxh = gkxh + mom_adj_h(gmzx, w, u, v), zxb = gkxb + 0.5 * w * gmzx[3];
+            const double zyh = gkyh + mom_adj_h(gmzy, w, u, v), zyb = gkyb + 0.5 * w * gmzy[3];
+            const double g_q = mt[0] * Ah + coeff * shh * (cx * zxh + cy * zyh);
+            const double g_qb = mt[0] * Ab + coeff * shb * (cx * zxb + cy * zyb);
+            const double phi[4] = {w, w * u, w * v, 0.5 * w * (u * u + v * v)};
+            double g_qt[4], g_qbt[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double fold = 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) fold += gJ[q * 4 + rr] * phi[rr];
+                g_qt[q] = -tdx[q] * gkxh - tdy[q] * gkyh + tnl[q] * gnlh + tnr[q] * gnrh + ttg[q] * gtgh +
+                          ttm[q] * mt[2] * Ah + fold;
+                g_qbt[q] = -tdx[q] * gkxb - tdy[q] * gkyb + tnl[q] * gnlb + tnr[q] * gnrb + ttg[q] * gtgb +
+                           ttm[q] * mt[2] * Ab + 0.5 * w * gJ[q * 4 + 3];
+            }
+            qtheta_adj(ck, th, i0, iT, u, v, mh, g_q, g_qb, g_qt, g_qbt, gth);
+        }
+        block_sum<4>(gth, lds);
+        double gw0[4];
+        solve4T(J, gth, gw0);
+        // scalar chain (w0, Pr, mu_ref) -> (rho, ux, vy, lam, coeff, tna, lama, m0..m4)
+        double gO[FC_N];
+        gO[FC_RHO] = R2[17]; gO[FC_UX] = R2[14]; gO[FC_VY] = R2[15]; gO[FC_LAM] = R2[13]; gO[FC_COEFF] = R2[12];
+        gO[FC_TNA] = R2[16]; gO[FC_LAMA] = R2[18];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) gO[FC_M0 + q] = R1[q];
+#pragma unroll
+        for (int dir = 0; dir < 6; ++dir) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < FC_N; ++q) s += gO[q] * jac[(1 + dir) * FJ_W + q];
+            if (dir < 4) gw0[dir] += s;
+            else if (dir == 4) g_pr += s;
+            else g_mu += s;
+        }
+        uni<4>(gw0);
+        // ---- write the record adjoints
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double vn = axis == 1 ? u : v, vt = axis == 1 ? v : u;
+            const double Gh = gwa * (sfac * (ghr[k] - ghl[k]) + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sfac * (gbr[k] - gbl[k]) + 0.5 * w * gfw[3]);
+            const double Ah = vn * Gh, Ab = vn * Gb;
+            const double cx = u - ux, cy = v - vy, c2 = cx * cx + cy * cy;
+            const double P = 0.5 * w * (g_q0x * cx + g_q0y * cy);
+            g6[k] = mt[3] * Ah + P * c2 + mom_adj_h(gw0, w, u, v);
+            g6[nv + k] = -mt[4] * vn * Ah;
+            g6[2 * nv + k] = -mt[4] * vt * Ah;
+            g6[3 * nv + k] = mt[3] * Ab + P + 0.5 * w * gw0[3];
+            g6[4 * nv + k] = -mt[4] * vn * Ab;
+            g6[5 * nv + k] = -mt[4] * vt * Ab;
+        }
+        if (threadIdx.x == 0) {
+            // macro slopes: mnl = ws_l(dn)/ln, mnr = ws_r(dn)/ln, mtg = (ws_l(dt) + ws_r(dt))/(2 lt)
+            for (int m = 0; m < 4; ++m) {
+                double* gl = D.a_ws + (size_t)(cl * 4 + m) * 24 + (sl * 3 + a) * 2;
+                double* gr = D.a_ws + (size_t)(cr * 4 + m) * 24 + (sr * 3 + a) * 2;
+                gl[0] = gml[m] / ln;
+                gr[0] = gmr[m] / ln;
+                gl[1] = 0.5 * gmt[m] / lt;
+                gr[1] = 0.5 * gmt[m] / lt;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        double* gp = D.a_par + 4 * ((size_t)D.ncell + f);
+        gp[0] = g_mu; gp[1] = g_pr; gp[2] = 0.0; gp[3] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// multiscale diffuse wall: one workgroup per wall face
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_adj_wall(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    UGKS_ADJ_LDS(NT, 24);
+    UGKS_DYN_SHARED(double, dyn);
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
+    const size_t nv = (size_t)nvel;
+    if ((int)blockIdx.x >= 2 * nx) return;
+    double* gmh = dyn;          // accumulated adjoint of q(theta) per velocity
+    double* gmb = dyn + nvel;
+    const int top = blockIdx.x >= nx ? 1 : 0, i = blockIdx.x % nx;
+    const double side = top ? 1.0 : -1.0;
+    const int c = top ? (ny - 1) * nx + i : i;
+    const double* rec = (top ? D.wt : D.wb) + (size_t)i * REC_ROWS * nv;
+    double* grec = (top ? D.a_wt : D.a_wb) + (size_t)i * REC_ROWS * nv;
+    const double* tape = D.wtape + (size_t)blockIdx.x * WT_LEN;
+    const double dt = D.scal[SC_DT], dn = D.dy, area = D.dx * D.dy;
+    const double ux = D.wall_ux[top], theta = 0.5 * D.wall_tn[top], lambda = 0.5 / theta;
+    const double sgn = top ? -1.0 : 1.0;   // sign of the wall flux in the wall cell's divergence
+    double gfw[4];
+    for (int m = 0; m < 4; ++m)
+        gfw[m] = uni(sgn * D.dx / area * D.a_Wn[4 * c + m] + D.a_gwall[(top * nx + i) * 4 + m]);
+    const double* gh_c = D.a_hs + (size_t)c * nv;
+    const double* gb_c = D.a_bs + (size_t)c * nv;
+    double th[4], J[16];
+    for (int q = 0; q < 4; ++q) th[q] = tape[q];
+    for (int q = 0; q < 16; ++q) J[q] = tape[4 + q];
+    const double T = th[3], i0 = 1.0 / th[0], iT = 1.0 / T, pre = th[0] / (UGKS_PI * T);
+    const double bfac = 0.5 * (double)ck * T;
+    const bool shakhov = D.gas.pr != 1.0;
+    double in[WI_N];
+    for (int m = 0; m < 4; ++m) in[WI_WC + m] = D.W[4 * c + m];
+    in[WI_UX] = ux; in[WI_MU] = D.gas.mu_ref; in[WI_PR] = D.gas.pr;
+    double gin[WI_N];
+    for (int q = 0; q < WI_N; ++q) gin[q] = 0.0;
+    UGKS_VLOOP(k, NT, nvel) { gmh[k] = 0.0; gmb[k] = 0.0; }
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        const double gwa = KA.gw[a];
+        const double* q6 = rec + (size_t)(a * 6) * nv;
+        double* g6 = grec + (size_t)(a * 6) * nv;
+        for (int q = 0; q < 6; ++q) in[WI_RED + q] = tape[20 + 8 * a + q];
+        const double inc = tape[26 + 8 * a], refl = tape[27 + 8 * a];
+        const double* jac = D.a_wjac + (size_t)(blockIdx.x * 3 + a) * 14 * WJ_W;
+        double sc[WC_N];
+        for (int q = 0; q < WC_N; ++q) sc[q] = jac[q];
+        const double* mt = sc + WC_MT;
+        const double* an = sc + WC_AN;
+        const double* at = sc + WC_AT;
+        const double* att = sc + WC_ATT;
+        const double coeff = sc[WC_COEFF], mxi2 = sc[WC_MXI2], irg = sc[WC_IRG];
+        const double hl0 = in[WI_RED + 4], hl1 = in[WI_RED + 5];
+        const double rhow = -inc / refl;
+        // R1: the reflected density and the ux-derivatives of the heat moments
+        double R1[4] = {0, 0, 0, 0};   // g_rhow, g_irg (incoming part), dhl0/dux, dhl1/dux
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double Gh = gwa * (sgn * D.dx / area * gh_c[k] + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sgn * D.dx / area * gb_c[k] + 0.5 * w * gfw[3]);
+            const double mh = q_of(th, pre, iT, u, v), mb = bfac * mh;
+            if (!(side * v > 0.0)) {
+                R1[0] += (Gh * mh + Gb * mb) * dt * v * irg;
+                R1[1] += (Gh * mh + Gb * mb) * dt * v * rhow;
+            }
+            const double vn = v, vtl = -u + ux, v2l = vn * vn + vtl * vtl;
+            const double hin = q6[k], bin = q6[3 * nv + k];
+            R1[2] += 0.5 * w * vn * 2.0 * vtl * hin;
+            R1[3] += 0.5 * w * ((v2l * hin + bin) + 2.0 * vtl * vtl * hin);
+        }
+        block_sum<4>(R1, lds);
+        const double g_inc = -R1[0] / refl, g_refl = R1[0] * inc / (refl * refl);
+        double g_irg = R1[1] + g_refl * refl / irg;
+        // R2: the outgoing-side distribution
+        double R2[22];   // mt(5) an(4) at(4) att(4) coeff hl0 hl1 ux mxi2
+        for (int q = 0; q < 22; ++q) R2[q] = 0.0;
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double Gh = gwa * (sgn * D.dx / area * gh_c[k] + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sgn * D.dx / area * gb_c[k] + 0.5 * w * gfw[3]);
+            const double mh = q_of(th, pre, iT, u, v), mb = bfac * mh;
+            const double vn = v, vt = -u, v2 = vn * vn + vt * vt;
+            if (side * v > 0.0) {
+                const double gh = v * Gh + g_inc * w * v, gb = v * Gb;
+                const double hin = q6[k], hn = q6[nv + k], ht = q6[2 * nv + k];
+                const double bin = q6[3 * nv + k], bn = q6[4 * nv + k], bt = q6[5 * nv + k];
+                double X[4];
+                for (int q = 0; q < 4; ++q) X[q] = mt[1] * vn * an[q] + mt[1] * vt * at[q] + mt[2] * att[q];
+                const double Lx = X[0] + X[1] * vn + X[2] * vt + 0.5 * X[3] * v2;
+                const double Eh[4] = {mh, vn * mh, vt * mh, 0.5 * (v2 * mh + mb)};
+                const double Eb[4] = {mb, vn * mb, vt * mb, 0.5 * (v2 * mb + mxi2 * mh)};
+                double E[4];
+                for (int q = 0; q < 4; ++q) E[q] = gh * Eh[q] + gb * Eb[q];
+                double corrh = 0.0, corrb = 0.0, qdot = 0.0, shh = 0.0, shb = 0.0, vtl = 0.0;
+                if (shakhov) {
+                    vtl = -u + ux;
+                    const double v2l = vn * vn + vtl * vtl;
+                    qdot = vn * hl0 + vtl * hl1;
+                    shh = 2.0 * lambda * v2l - 4.0;
+                    shb = shh + 2.0 / (double)ck;
+                    corrh = coeff * qdot * shh * mh;
+                    corrb = coeff * qdot * shb * mb;
+                }
+                R2[0] += gh * (mh + corrh) + gb * (mb + corrb);
+                double e_an = 0.0, e_at = 0.0, e_att = 0.0;
+                for (int q = 0; q < 4; ++q) { e_an += E[q] * an[q]; e_at += E[q] * at[q]; e_att += E[q] * att[q]; }
+                R2[1] += vn * e_an + vt * e_at;
+                R2[2] += e_att;
+                R2[3] += gh * hin + gb * bin;
+                R2[4] -= gh * (vn * hn + u * ht) + gb * (vn * bn + u * bt);
+                for (int q = 0; q < 4; ++q) {
+                    R2[5 + q] += mt[1] * vn * E[q];
+                    R2[9 + q] += mt[1] * vt * E[q];
+                    R2[13 + q] += mt[2] * E[q];
+                }
+                double gmhk = gh * (mt[0] + Lx) + gb * (0.5 * X[3] * mxi2);
+                double gmbk = gh * (0.5 * X[3]) + gb * (mt[0] + Lx);
+                if (shakhov) {
+                    R2[17] += mt[0] * qdot * (gh * shh * mh + gb * shb * mb);
+                    const double gq = mt[0] * coeff * (gh * shh * mh + gb * shb * mb);
+                    R2[18] += gq * vn;
+                    R2[19] += gq * vtl;
+                    const double gs = mt[0] * coeff * qdot * (gh * mh + gb * mb);
+                    R2[20] += gq * hl1 + gs * 4.0 * lambda * vtl;
+                    gmhk += gh * mt[0] * coeff * qdot * shh;
+                    gmbk += gb * mt[0] * coeff * qdot * shb;
+                }
+                R2[21] += gb * 0.5 * X[3] * mh;
+                gmh[k] += gmhk;
+                gmb[k] += gmbk;
+            } else {
+                gmh[k] += Gh * dt * v * rhow * irg + g_refl * dt * w * v * irg;
+                gmb[k] += Gb * dt * v * rhow * irg;
+            }
+        }
+        block_sum<22>(R2, lds);
+        // scalar chain reverse for this node's outputs
+        double gO[WC_N];
+        for (int q = 0; q < WC_N; ++q) gO[q] = 0.0;
+        for (int q = 0; q < 5; ++q) gO[WC_MT + q] = R2[q];
+        for (int q = 0; q < 4; ++q) { gO[WC_AN + q] = R2[5 + q]; gO[WC_AT + q] = R2[9 + q]; gO[WC_ATT + q] = R2[13 + q]; }
+        gO[WC_COEFF] = R2[17];
+        gO[WC_MXI2] = R2[21];
+        gO[WC_IRG] = g_irg;
+        double gred[6] = {0, 0, 0, 0, R2[18], R2[19]};
+        double g_ux = R2[20];
+#pragma unroll
+        for (int dir = 0; dir < WI_N; ++dir) {
+            double s = 0.0;
+            for (int q = 0; q < WC_N; ++q) s += gO[q] * jac[(1 + dir) * WJ_W + q];
+            if (dir >= WI_RED) gred[dir - WI_RED] += s;
+            else gin[dir] += s;
+        }
+        g_ux += gred[4] * R1[2] + gred[5] * R1[3];
+        gin[WI_UX] += g_ux;
+        // record adjoints of this node
+        UGKS_VLOOP(k, NT, nvel) {
+            const double u = D.u[k], v = D.v[k], w = D.w[k];
+            const double Gh = gwa * (sgn * D.dx / area * gh_c[k] + mom_adj_h(gfw, w, u, v));
+            const double Gb = gwa * (sgn * D.dx / area * gb_c[k] + 0.5 * w * gfw[3]);
+            const double vn = v, vt = -u, v2 = vn * vn + vt * vt;
+            const double vtl = -u + ux, v2l = vn * vn + vtl * vtl;
+            double gh = 0.0, gb = 0.0;
+            if (side * v > 0.0) { gh = v * Gh + g_inc * w * v; gb = v * Gb; }
+            const double hs = 0.5 * w * (gred[4] * vn + gred[5] * vtl);
+            g6[k] = mt[3] * gh + hs * v2l;
+            g6[nv + k] = -mt[4] * vn * gh;
+            g6[2 * nv + k] = -mt[4] * u * gh - w * (gred[0] + gred[1] * vn + gred[2] * vt + 0.5 * gred[3] * v2);
+            g6[3 * nv + k] = mt[3] * gb + hs;
+            g6[4 * nv + k] = -mt[4] * vn * gb;
+            g6[5 * nv + k] = -mt[4] * u * gb - 0.5 * w * gred[3];
+        }
+    }
+    // the closure of the wall equilibrium ww(W, ux)
+    double gth[4] = {0, 0, 0, 0};
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k];
+        const double mh = q_of(th, pre, iT, u, v);
+        const double z[4] = {0, 0, 0, 0};
+        qtheta_adj(ck, th, i0, iT, u, v, mh, gmh[k], gmb[k], z, z, gth);
+    }
+    block_sum<4>(gth, lds);
+    double gww[4];
+    solve4T(J, gth, gww);
+    {
+        double gO[WC_N];
+        for (int q = 0; q < WC_N; ++q) gO[q] = 0.0;
+        for (int q = 0; q < 4; ++q) gO[WC_WW + q] = gww[q];
+        const double* jac = D.a_wjac + (size_t)(blockIdx.x * 3) * 14 * WJ_W;
+#pragma unroll
+        for (int dir = 0; dir < WI_RED; ++dir) {
+            double s = 0.0;
+            for (int q = 0; q < WC_N; ++q) s += gO[q] * jac[(1 + dir) * WJ_W + q];
+            gin[dir] += s;
+        }
+    }
+    if (threadIdx.x == 0) {
+        for (int m = 0; m < 4; ++m) D.a_Wwall[4 * blockIdx.x + m] = gin[WI_WC + m];
+        double* gp = D.a_par + 4 * ((size_t)D.ncell + D.ncell + nx * (ny - 1) + blockIdx.x);
+        gp[0] = gin[WI_MU]; gp[1] = gin[WI_PR];
+        gp[2] = top ? 0.0 : gin[WI_UX];
+        gp[3] = top ? gin[WI_UX] : 0.0;
+        // the wall side of the wall cell carries no macro slope into any flux
+        const int sd = top ? 3 : 2;
+        for (int m = 0; m < 4; ++m)
+            for (int a = 0; a < 3; ++a) {
+                double* g = D.a_ws + (size_t)(c * 4 + m) * 24 + (sd * 3 + a) * 2;
+                g[0] = 0.0;
+                g[1] = 0.0;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// limiter and reconstruction of h and b: two lanes per (cell, velocity)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t >> 1, fld = t & 1;
+    const int c = blockIdx.y;
+    if (c >= D.ncell) return;
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
+    const size_t nv = (size_t)nvel;
+    const int i = c % nx, j = c / nx;
+    int rb, sh;
+    bool near;
+    stencil_rows(j, ny, &rb, &near, &sh);
+    const double uk = D.u[k], vk = D.v[k];
+    const int xs = uk >= 0.0 ? 1 : 0;
+    const int ys = vk >= 0.0 ? 3 : 2;
+    const int wsd = j == 0 ? 2 : (j == ny - 1 ? 3 : -1);
+    const double* F = fld == 0 ? D.h : D.b;
+    double fv[4][3], vol[3][3];
+    SideOut xo, yo, wo;
+    const double m = F[(size_t)c * nv + k];
+    recon_field(KA, D.lines, nx, nvel, i, j, rb, near, sh, t, xs, ys, wsd, fv, vol, xo, yo, wo);
+    int arg;
+    const double th_lane = theta_arg(m, fv, vol, &arg);
+    const double th_part = lane_xor1(th_lane);
+    const double m_other = lane_xor1(m);
+    const bool zeroed = m <= 0.0 || m_other <= 0.0;
+    double theta = fmin(th_lane, th_part);
+    if (zeroed) theta = 0.0;
+    theta = fmax(0.0, fmin(1.0, theta));
+    const double idx = 1.0 / D.dx, idy = 1.0 / D.dy;
+    const int off = 3 * fld;
+    // record adjoints of the sides this cell supplied
+    const double* gx = D.a_xf + (size_t)(xs == 1 ? c : j * nx + (i == 0 ? nx - 1 : i - 1)) * REC_ROWS * nv + k;
+    const double* gy = nullptr;
+    if (ys == 3 && j < ny - 1) gy = D.a_yf + (size_t)c * REC_ROWS * nv + k;
+    if (ys == 2 && j > 0) gy = D.a_yf + (size_t)(c - nx) * REC_ROWS * nv + k;
+    const double* gw = nullptr;
+    if (j == 0) gw = D.a_wb + (size_t)i * REC_ROWS * nv + k;
+    if (j == ny - 1) gw = D.a_wt + (size_t)i * REC_ROWS * nv + k;
+    double gside[4][3][3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) gside[s][a][e] = 0.0;
+    double gm = 0.0, gth = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        // x side
+        {
+            const double g0 = gx[(a * 6 + off) * nv], g1 = gx[(a * 6 + off + 1) * nv], g2 = gx[(a * 6 + off + 2) * nv];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (s == xs) {
+                    gside[s][a][0] += theta * g0;
+                    gside[s][a][1] += theta * idx * g1;
+                    gside[s][a][2] += theta * idy * g2;
+                }
+            gm += (1.0 - theta) * g0;
+            gth += (xo.d[a][0] - m) * g0 + xo.d[a][1] * idx * g1 + xo.d[a][2] * idy * g2;
+        }
+        if (gy) {
+            const double g0 = gy[(a * 6 + off) * nv], g1 = gy[(a * 6 + off + 1) * nv], g2 = gy[(a * 6 + off + 2) * nv];
+#pragma unroll
+            for (int s = 2; s < 4; ++s)
+                if (s == ys) {
+                    gside[s][a][0] += theta * g0;
+                    gside[s][a][1] += theta * idy * g1;
+                    gside[s][a][2] += theta * idx * g2;
+                }
+            gm += (1.0 - theta) * g0;
+            gth += (yo.d[a][0] - m) * g0 + yo.d[a][1] * idy * g1 + yo.d[a][2] * idx * g2;
+        }
+        if (gw) {
+            const double g0 = gw[(a * 6 + off) * nv], g1 = gw[(a * 6 + off + 1) * nv], g2 = gw[(a * 6 + off + 2) * nv];
+#pragma unroll
+            for (int s = 2; s < 4; ++s)
+                if (s == wsd) {
+                    gside[s][a][0] += theta * g0;
+                    gside[s][a][1] += theta * idy * g1;
+                    gside[s][a][2] += theta * idx * g2;
+                }
+            gm += (1.0 - theta) * g0;
+            gth += (wo.d[a][0] - m) * g0 + wo.d[a][1] * idy * g1 + wo.d[a][2] * idx * g2;
+        }
+    }
+    gth += lane_xor1(gth);
+    double gvol[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) gvol[a][b] = 0.0;
+    const bool mine = th_lane < th_part || (th_lane == th_part && fld == 0);
+    if (!zeroed && theta > 0.0 && theta < 1.0 && mine && arg >= 0) {
+        double f = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (arg == s * 3 + a) f = fv[s][a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                if (arg == 12 + a * 3 + b) f = vol[a][b];
+        const double keep = 1.0 - 64.0 * UGKS_DBL_EPS, den = m - f;
+        gm += gth * keep * (-f) / (den * den);
+        const double gf = gth * keep * m / (den * den);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (arg == s * 3 + a) gside[s][a][0] += gf;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                if (arg == 12 + a * 3 + b) gvol[a][b] += gf;
+    }
+    // second stage reversed from the first-stage lines, then the first stage
+    const size_t row = 2 * nv;
+    FirstLayer L, gL;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double* p = D.lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
+        L.xv[r][0] = p[0]; L.xv[r][1] = p[row]; L.xd[r][0] = p[2 * row]; L.xd[r][1] = p[3 * row];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) L.rows[r][a] = p[(4 + a) * row];
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        int col = i + s - 2;
+        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+        const double* p = D.lines + (size_t)(j * nx + col) * LINE_ROWS * row + t;
+        L.yv[s][0] = p[7 * row]; L.yv[s][1] = p[8 * row]; L.yd[s][0] = p[9 * row]; L.yd[s][1] = p[10 * row];
+    }
+    zero_layer(gL);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) side_adj(KA, L, near, sh, s, gside[s], gL);
+    vol_adj(KA, L, near, sh, gvol, gL);
+    double blk[5][5], gblk[5][5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            int col = i + s - 2;
+            col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+            blk[r][s] = F[(size_t)((rb + r) * nx + col) * nv + k];
+            gblk[r][s] = 0.0;
+        }
+    first_layer_adj(KA, blk, near, sh, gL, gblk);
+    gblk[j - rb][2] += gm;
+    double* o = D.a_blk + (size_t)c * 25 * row + t;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) o[(size_t)(r * 5 + s) * row] = gblk[r][s];
+}
+
+// lam_h, lam_b: direct part + streamed part + gathered block adjoints
+__global__ void __launch_bounds__(256) k_adj_gather(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t >> 1, fld = t & 1, c = blockIdx.y;
+    if (c >= D.ncell || k >= D.nvel) return;
+    const size_t nv = (size_t)D.nvel, o = (size_t)c * nv + k;
+    const int i = c % D.nx, j = c / D.nx;
+    double* out = fld == 0 ? D.a_h : D.a_b;
+    const double* st = fld == 0 ? D.a_hs : D.a_bs;
+    out[o] = out[o] + st[o] + gather_block(D.a_blk, 2 * nv, (size_t)t, D.nx, D.ny, i, j);
+}
+
+// parameter gradients: fixed-order sum of the per-cell, per-face and
+// per-wall partials (one workgroup)
+__global__ void __launch_bounds__(256) k_adj_sum(const Dev* __restrict__ DS, int nslots) {
+    const Dev& D = DS[blockIdx.z];
+    __shared__ double lds[16 * 4];
+    double s[4] = {0, 0, 0, 0};
+    for (int q = threadIdx.x; q < nslots; q += blockDim.x)
+        for (int m = 0; m < 4; ++m) s[m] += D.a_par[4 * (size_t)q + m];
+    block_sum<4>(s, lds);
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 4; ++m) D.a_sum[m] = s[m];
+}
+
+// ---------------------------------------------------------------------------
+// launcher: the reverse stages after a forward step with the tapes on
+// ---------------------------------------------------------------------------
+template <class K>
+static hipError_t adj_allow_lds(K kernel, size_t bytes, size_t* done) {
+    if (bytes <= *done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes);
+    if (e == hipSuccess) *done = bytes;
+    return e;
+}
+
+extern "C" hipError_t ugks_launch_wrecon(const LaunchSet* L, int proposal, hipStream_t s, Prof* P);
+
+template <int NT>
+static hipError_t launch_adjoint(const LaunchSet& L, hipStream_t s, Prof* P) {
+    static size_t lds_wall = 0;
+    const Dev& D = *L.size;
+    const int n = L.n;
+    const size_t q2 = 2 * sizeof(double) * D.nvel;
+    hipError_t e;
+    if ((e = adj_allow_lds(k_adj_wall<NT>, q2, &lds_wall)) != hipSuccess) return e;
+    const dim3 gw((4 * D.nx + 255) / 256, D.ny, n);
+    const int nfaces = D.ncell + D.nx * (D.ny - 1);
+    UGKS_TIMED(P, K_ADJ_CELL, s, hipLaunchKernelGGL(k_adj_cn_cell<NT>, dim3(D.ncell, 1, n), dim3(NT), 0, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_TARGETS, s,
+               hipLaunchKernelGGL(k_adj_targets<NT>, dim3(D.ncell, 1, n), dim3(NT), 0, s, L.dev, 1));
+    UGKS_TIMED(P, K_ADJ_WRECON, s, hipLaunchKernelGGL(k_adj_wrecon, gw, dim3(256), 0, s, L.dev, 1));
+    UGKS_TIMED(P, K_ADJ_WGATHER, s,
+               hipLaunchKernelGGL(k_adj_wgather, dim3((4 * D.ncell + 255) / 256, 1, n), dim3(256), 0, s, L.dev, 1));
+    UGKS_TIMED(P, K_ADJ_TARGETS, s,
+               hipLaunchKernelGGL(k_adj_targets<NT>, dim3(D.ncell, 1, n), dim3(NT), 0, s, L.dev, 0));
+    if ((e = ugks_launch_wrecon(&L, 0, s, P)) != hipSuccess) return e;
+    const int nchain = nfaces * 3 * 7 + 2 * D.nx * 3 * 14;
+    UGKS_TIMED(P, K_ADJ_CHAINS, s,
+               hipLaunchKernelGGL(k_adj_chains, dim3((nchain + 255) / 256, 1, n), dim3(256), 0, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_FACE, s, hipLaunchKernelGGL(k_adj_face<NT>, dim3(nfaces, 1, n), dim3(NT), 0, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_WALL, s, hipLaunchKernelGGL(k_adj_wall<NT>, dim3(2 * D.nx, 1, n), dim3(NT), q2, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_WRECON, s, hipLaunchKernelGGL(k_adj_wrecon, gw, dim3(256), 0, s, L.dev, 0));
+    UGKS_TIMED(P, K_ADJ_WGATHER, s,
+               hipLaunchKernelGGL(k_adj_wgather, dim3((4 * D.ncell + 255) / 256, 1, n), dim3(256), 0, s, L.dev, 0));
+    UGKS_TIMED(P, K_ADJ_RECON, s,
+               hipLaunchKernelGGL(k_adj_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_GATHER, s,
+               hipLaunchKernelGGL(k_adj_gather, dim3((2 * D.nvel + 255) / 256, D.ncell, n), dim3(256), 0, s, L.dev));
+    const int nslots = D.ncell + nfaces + 2 * D.nx;
+    UGKS_TIMED(P, K_ADJ_SUM, s, hipLaunchKernelGGL(k_adj_sum, dim3(1, 1, n), dim3(256), 0, s, L.dev, nslots));
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ugks_launch_cn_adjoint(const LaunchSet* L, hipStream_t s, Prof* P) {
+    switch (L->size->wg_vel) {
+    case 64: return launch_adjoint<64>(*L, s, P);
+    case 128: return launch_adjoint<128>(*L, s, P);
+    case 256: return launch_adjoint<256>(*L, s, P);
+    default: return hipErrorInvalidValue;
+    }
+}

@@ -19,6 +19,8 @@
 #include "ugks_recon.h"
 
 extern "C" hipError_t ugks_set_consts(const Consts* k);
+extern "C" hipError_t ugks_set_consts_adjoint(const Consts* k);
+extern "C" hipError_t ugks_launch_cn_adjoint(const LaunchSet* L, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_implicit(const LaunchSet* L, int rescale, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_wall_reflection(const LaunchSet* L, hipStream_t s, Prof* P);
@@ -59,6 +61,7 @@ struct Instance {
     Prof prof;
     bool cn_alloc = false;        // S2CN2 target buffers allocated
     bool cn_targets_fresh = false; // tg holds the volume targets of the current state
+    bool adj_alloc = false;       // S2CN2 tapes and reverse-step buffers allocated
     int lo_wg = 256;              // workgroup size of the low-order Explicit kernels
 };
 
@@ -121,6 +124,7 @@ int build_consts() {
         memcpy(K.wallmap[sh], m, sizeof m);
     }
     HIPCHK(ugks_set_consts(&K));
+    HIPCHK(ugks_set_consts_adjoint(&K));
     return E_OK;
 }
 
@@ -407,21 +411,28 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
 // S2CN2 explicit step (no counterpart as one call in the reference: it is
 // S2NativeEvolution's fluxes followed by S2NativeApplyUpdate, i.e. advance_step
 // with the conservation gate, src/mod_S2NativeAdapter.f90:469-599,867-1018).
+// S2CN2 target buffers, allocated on the first S2CN2 call
+static int cn_buffers(Instance* I) {
+    if (I->cn_alloc) return E_OK;
+    Dev& D = I->D;
+    const size_t cv = (size_t)D.ncell * D.nvel;
+    int rc;
+    for (int q = 0; q < 6; ++q)
+        if ((rc = dmalloc(I, &D.tg[q], cv)) || (rc = dmalloc(I, &D.tgn[q], cv))) return rc;
+    if ((rc = dmalloc(I, &D.cnpart, (size_t)D.ncell * 8))) return rc;
+    if ((rc = upload(I))) return rc;
+    I->cn_alloc = true;
+    I->cn_targets_fresh = false;
+    return E_OK;
+}
+
 extern "C" int ugks_cn_step(void* handle, int* accepted, double* diag16) {
     Instance* I = (Instance*)handle;
     if (!I || !accepted) return fail(E_ARG, "null argument");
     *accepted = 0;
     Dev& D = I->D;
-    if (!I->cn_alloc) {
-        const size_t cv = (size_t)D.ncell * D.nvel;
-        int rc;
-        for (int q = 0; q < 6; ++q)
-            if ((rc = dmalloc(I, &D.tg[q], cv)) || (rc = dmalloc(I, &D.tgn[q], cv))) return rc;
-        if ((rc = dmalloc(I, &D.cnpart, (size_t)D.ncell * 8))) return rc;
-        if ((rc = upload(I))) return rc;
-        I->cn_alloc = true;
-        I->cn_targets_fresh = false;
-    }
+    int rc0 = cn_buffers(I);
+    if (rc0) return rc0;
     LaunchSet L = one(I);
     HIPCHK(ugks_launch_cn(&L, I->cn_targets_fresh ? 0 : 1, I->stream, &I->prof));
     double sc[SC_COUNT];
@@ -456,6 +467,70 @@ extern "C" int ugks_cn_ledger(void* handle, double* out8) {
     HIPCHK(hipMemcpyAsync(sc, I->D.scal, sizeof sc, hipMemcpyDeviceToHost, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));
     for (int m = 0; m < 4; ++m) { out8[m] = sc[SC_CN_TOTAL0 + m]; out8[4 + m] = sc[SC_CN_BAL + m]; }
+    return E_OK;
+}
+
+// S2CN2 tapes and reverse-step buffers, allocated on the first reverse step
+static int adj_buffers(Instance* I) {
+    if (I->adj_alloc) return E_OK;
+    Dev& D = I->D;
+    const size_t nc = D.ncell, nv = D.nvel, cv = nc * nv, nx = D.nx, ny = D.ny;
+    const size_t nfaces = nc + nx * (ny - 1);
+    int rc;
+    struct { double** p; size_t n; } bufs[] = {
+        {&D.ftape, nfaces * 3 * FT_LEN}, {&D.wtape, 2 * nx * WT_LEN}, {&D.ttape, nc * 2 * 9 * TT_LEN},
+        {&D.ctape, nc * CT_LEN}, {&D.a_lamW, 4 * nc}, {&D.a_lamh, cv}, {&D.a_lamb, cv}, {&D.a_hs, cv},
+        {&D.a_bs, cv}, {&D.a_xf, nc * REC_ROWS * nv}, {&D.a_yf, nx * (ny - 1) * REC_ROWS * nv},
+        {&D.a_wb, nx * REC_ROWS * nv}, {&D.a_wt, nx * REC_ROWS * nv}, {&D.a_ws, nc * 96}, {&D.a_wv, nc * 36},
+        {&D.a_blk, nc * 25 * 2 * nv}, {&D.a_blkw, nc * 100}, {&D.a_W, 4 * nc}, {&D.a_Wn, 4 * nc},
+        {&D.a_Wwall, 8 * nx}, {&D.a_gwall, 8 * nx}, {&D.a_h, cv}, {&D.a_b, cv}, {&D.a_Wout, 4 * nc},
+        {&D.a_par, 4 * (nc + nfaces + 2 * nx)}, {&D.a_sum, 4}, {&D.a_fjac, nfaces * 3 * 7 * FJ_W},
+        {&D.a_wjac, 2 * nx * 3 * 14 * WJ_W}};
+    for (auto& q : bufs)
+        if ((rc = dmalloc(I, q.p, q.n))) return rc;
+    for (int q = 0; q < 6; ++q)
+        if ((rc = dmalloc(I, &D.a_tg[q], cv)) || (rc = dmalloc(I, &D.a_tgn[q], cv))) return rc;
+    if ((rc = upload(I))) return rc;
+    I->adj_alloc = true;
+    return E_OK;
+}
+
+// Reverse step of one S2CN2 step (no counterpart in the reference).  The
+// state S_k and the adjoints are device arrays in the ugks_get_state layout;
+// gwall (host, [2][nx][4]) is the adjoint of the step's wall fluxes.
+extern "C" int ugks_cn_adjoint_device(void* handle, const double* W, const double* h, const double* b,
+                                      const double* lamW, const double* lamh, const double* lamb,
+                                      const double* gwall, double* gW, double* gh, double* gb, double* gpar4) {
+    Instance* I = (Instance*)handle;
+    if (!I || !W || !h || !b || !lamW || !lamh || !lamb || !gwall || !gW || !gh || !gb || !gpar4)
+        return fail(E_ARG, "null argument");
+    int rc;
+    if ((rc = cn_buffers(I)) || (rc = adj_buffers(I))) return rc;
+    Dev& D = I->D;
+    const size_t nc = D.ncell, cv = nc * D.nvel;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(D.W, W, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.h, h, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.b, b, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamW, lamW, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamh, lamh, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamb, lamb, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_gwall, gwall, sizeof(double) * 8 * D.nx, hipMemcpyHostToDevice, I->stream));
+    double scal[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(scal, D.scal, sizeof scal, hipMemcpyDeviceToHost, I->stream));
+    I->operator_fresh = false;
+    I->cn_targets_fresh = false;
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_cn(&L, 1, I->stream, &I->prof));
+    if ((rc = check_status(I, nullptr))) return rc;
+    HIPCHK(ugks_launch_cn_adjoint(&L, I->stream, &I->prof));
+    HIPCHK(hipMemcpyAsync(gW, D.a_Wout, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(gh, D.a_h, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(gb, D.a_b, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(gpar4, D.a_sum, sizeof(double) * 4, hipMemcpyDeviceToHost, I->stream));
+    // the forward re-run must not advance the S2CN2 conservation ledger
+    HIPCHK(hipMemcpyAsync(D.scal, scal, sizeof scal, hipMemcpyHostToDevice, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
     return E_OK;
 }
 
@@ -546,10 +621,20 @@ extern "C" int ugks_debug_fetch(void* handle, int which, double* out, long n) {
     if (!I || !out) return fail(E_ARG, "null argument");
     const Dev& D = I->D;
     const size_t nv = D.nvel, nc = D.ncell;
-    const double* src[] = {D.xf, D.yf, D.wb, D.wt, D.ws, D.wv, D.scal};
-    const size_t len[] = {nc * REC_ROWS * nv, (size_t)D.nx * (D.ny - 1) * REC_ROWS * nv, (size_t)D.nx * REC_ROWS * nv,
-                          (size_t)D.nx * REC_ROWS * nv, nc * 96, nc * 36, SC_COUNT};
-    if (which < 0 || which > 6) return fail(E_ARG, "unknown buffer");
+    const size_t nfaces = nc + (size_t)D.nx * (D.ny - 1);
+    // 0-6 forward buffers; 7-30 reverse-step buffers (null until the first
+    // ugks_cn_adjoint_device call)
+    const double* src[] = {D.xf, D.yf, D.wb, D.wt, D.ws, D.wv, D.scal,
+                           D.a_hs, D.a_bs, D.a_tg[0], D.a_tg[1], D.a_tg[2], D.a_tg[3], D.a_tg[4], D.a_tg[5],
+                           D.a_tgn[0], D.a_tgn[1], D.a_tgn[2], D.a_tgn[3], D.a_tgn[4], D.a_tgn[5],
+                           D.a_Wn, D.a_W, D.a_xf, D.a_yf, D.a_wb, D.a_wt, D.a_ws, D.a_wv, D.a_par, D.a_Wwall};
+    const size_t cv = nc * nv, rx = (size_t)D.nx * REC_ROWS * nv;
+    const size_t len[] = {nc * REC_ROWS * nv, (size_t)D.nx * (D.ny - 1) * REC_ROWS * nv, rx, rx, nc * 96, nc * 36,
+                          SC_COUNT, cv, cv, cv, cv, cv, cv, cv, cv, cv, cv, cv, cv, cv, cv, 4 * nc, 4 * nc,
+                          nc * REC_ROWS * nv, (size_t)D.nx * (D.ny - 1) * REC_ROWS * nv, rx, rx, nc * 96, nc * 36,
+                          4 * (nc + nfaces + 2 * (size_t)D.nx), 8 * (size_t)D.nx};
+    const int nbuf = (int)(sizeof len / sizeof len[0]);
+    if (which < 0 || which >= nbuf || !src[which]) return fail(E_ARG, "unknown buffer");
     size_t m = std::min((size_t)n, len[which]);
     HIPCHK(hipMemcpyAsync(out, src[which], sizeof(double) * m, hipMemcpyDeviceToHost, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));

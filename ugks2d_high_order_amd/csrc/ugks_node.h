@@ -1,0 +1,346 @@
+// Device pieces of the S2 operator shared by the forward kernels and the
+// reverse (adjoint) kernels: the distribution reconstruction's second stage
+// and limiter, the discrete-equilibrium Newton closure, the multiscale wall
+// helpers.  Each translation unit that includes it passes its own copy of
+// the constant tables (Consts) where they are needed.
+#pragma once
+#include "ugks_recon.h"
+#include "ugks_state.h"
+
+// ===========================================================================
+// micro reconstruction: one thread per (cell, velocity), h and b together
+// ===========================================================================
+struct SideOut {
+    double d[3][3];   // node x {value, dn, dt}
+};
+
+// Second stage of one field at cell (i, j): the FirstLayer gathered from the
+// lines of the column stencil (rows rb..rb+4 at column i) and of the row
+// stencil (columns i-2..i+2 of row j); t = 2k + field within the cell.
+__device__ inline void recon_field(const Consts& K, const double* lines, int nx, int nvel, int i, int j, int rb, bool near, int sh,
+                                   int t, int xs, int ys, int wsd, double fval[4][3], double vol[3][3], SideOut& xo,
+                                   SideOut& yo, SideOut& wo) {
+    const size_t row = 2 * (size_t)nvel;
+    FirstLayer L;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double* p = lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
+        L.xv[r][0] = p[0]; L.xv[r][1] = p[row]; L.xd[r][0] = p[2 * row]; L.xd[r][1] = p[3 * row];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) L.rows[r][a] = p[(4 + a) * row];
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        int col = i + s - 2;
+        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+        const double* p = lines + (size_t)(j * nx + col) * LINE_ROWS * row + t;
+        L.yv[s][0] = p[7 * row]; L.yv[s][1] = p[8 * row]; L.yd[s][0] = p[9 * row]; L.yd[s][1] = p[10 * row];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        double d[3][3];
+        side_data(K, L, near, sh, s, d);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            fval[s][a] = d[a][0];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (s == xs) xo.d[a][q] = d[a][q];
+                if (s == ys) yo.d[a][q] = d[a][q];
+                if (s == wsd) wo.d[a][q] = d[a][q];
+            }
+        }
+    }
+    vol_data(K, L, near, sh, vol);
+}
+
+__device__ inline double theta_of(double m, double fval[4][3], double vol[3][3], double theta) {
+    const double keep = 1.0 - 64.0 * UGKS_DBL_EPS;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (fval[s][a] < 0.0) theta = fmin(theta, keep * m / (m - fval[s][a]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (vol[a][b] < 0.0) theta = fmin(theta, keep * m / (m - vol[a][b]));
+    return theta;
+}
+
+// write one limited side of one field (rows off..off+2 of each node) into a
+// face record; isn, ist: reciprocals of the normal and tangential cell sizes
+__device__ inline void store_side(double* rec, size_t nv, int off, double theta, double m, const SideOut& so,
+                                  double isn, double ist) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double* p = rec + (size_t)(a * 6 + off) * nv;
+        p[0] = m + theta * (so.d[a][0] - m);
+        p[nv] = theta * so.d[a][1] * isn;
+        p[2 * nv] = theta * so.d[a][2] * ist;
+    }
+}
+
+// ===========================================================================
+// velocity-parallel closure machinery
+// ===========================================================================
+// Velocity loops: thread t of an NT-thread workgroup owns velocities t, t+NT,
+// t+2NT, ...  Per-velocity inputs (velocity table, records, states) are read
+// from memory in every pass and stay in the caches, so nothing per velocity is
+// held in registers across a workgroup reduction.  Per-velocity values shared
+// by several passes live in LDS slots indexed by velocity; a slot is only ever
+// touched by the thread that owns the velocity, so the slots need no barriers.
+#define UGKS_VLOOP(k, NT, n) _Pragma("unroll 1") for (int k = threadIdx.x; k < (n); k += (NT))
+
+// q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q: the moments of q and of
+// dq/dtheta_a (Jacobian columns), red[0..3] moments, red[4+4a..] column a.
+template <int NT>
+__device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds) {
+#pragma unroll
+    for (int a = 0; a < 20; ++a) red[a] = 0.0;
+    const double r0 = th[0], T = uni(th[3]), t1 = uni(th[1]), t2 = uni(th[2]);
+    const double pre = uni(r0 / (UGKS_PI * T));
+    const double i0 = uni(1.0 / r0), iT = uni(1.0 / T);
+    const int nvel = D.nvel;
+    UGKS_VLOOP(k, NT, nvel) {
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
+        double du = u - t1, dv = v - t2, c2 = du * du + dv * dv;
+        double q = pre * exp(-c2 * iT);
+        double qb = 0.5 * (double)ck * T * q;
+        double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
+        double lb3 = l3 + iT;
+        mom_add(red + 0, w, u, v, q, qb);
+        mom_add(red + 4, w, u, v, q * i0, qb * i0);
+        mom_add(red + 8, w, u, v, q * l1, qb * l1);
+        mom_add(red + 12, w, u, v, q * l2, qb * l2);
+        mom_add(red + 16, w, u, v, q * l3, qb * lb3);
+    }
+    block_sum<20>(red, lds);
+}
+
+// q(theta) of one velocity (the expression q_pass uses); iT = 1/T.  Hot
+// loops multiply by reciprocals of the uniform T, rho instead of dividing.
+__device__ inline double q_of(const double th[4], double pre, double iT, double u, double v) {
+    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+    return pre * exp(-c2 * iT);
+}
+
+__device__ inline double scaled_res(const double r[4], const double w[4]) {
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) m = fmax(m, fabs(r[a]) / fmax(1.0, fabs(w[a])));
+    return m;
+}
+
+// Newton solve for the discrete conservative equilibrium with moments w
+// (halving line search down to 2^-20, at most 12 steps).  On success th and
+// J describe the solution and, with STORE, qh[k] (LDS slots of the owned
+// velocities) holds q(th), all positive and finite.
+// With STORE false the caller computes q itself and must check positivity.
+template <int NT, bool STORE>
+__device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], double th[4], double J[16], double* qh,
+                                     double* lds) {
+    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || !(w[0] > 0.0)) return false;
+    th[0] = w[0];
+    th[1] = w[1] / w[0];
+    th[2] = w[2] / w[0];
+    th[3] = 4.0 * (w[3] / w[0] - 0.5 * (th[1] * th[1] + th[2] * th[2])) / (double)(ck + 2);
+    if (!(th[3] > 0.0) || !isfinite(th[3])) return false;
+    double red[20], res[4];
+    q_pass<NT>(ck, D, th, red, lds);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
+    double nrm = scaled_res(res, w);
+    for (int it = 0; it < 12; ++it) {
+        if (nrm <= UGKS_NEWTON_TOL) break;
+        double step[4], mres[4] = {-res[0], -res[1], -res[2], -res[3]};
+        if (!solve4(J, mres, step)) return false;
+        // line search: J and res are rebuilt from the accepted trial, so
+        // neither is live across the trial passes
+        double alpha = 1.0, nn = 0.0;
+        bool acc = false;
+        while (alpha >= 9.5367431640625e-07) {
+            double nt[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) nt[a] = th[a] + alpha * step[a];
+            if (nt[0] > 0.0 && nt[3] > 0.0 && isfinite(nt[0]) && isfinite(nt[1]) && isfinite(nt[2]) &&
+                isfinite(nt[3])) {
+                q_pass<NT>(ck, D, nt, red, lds);
+                bool fin = true;
+#pragma unroll
+                for (int a = 0; a < 20; ++a) fin = fin && isfinite(red[a]);
+                if (fin) {
+                    double nres[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) nres[a] = red[a] - w[a];
+                    nn = scaled_res(nres, w);
+                    if (nn < nrm || nn <= UGKS_NEWTON_TOL) {
+                        acc = true;
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) th[a] = nt[a];
+                    }
+                }
+                if (acc) break;
+            }
+            alpha *= 0.5;
+        }
+        if (!acc) return false;
+#pragma unroll
+        for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
+        nrm = nn;
+    }
+    if (!(nrm <= UGKS_NEWTON_TOL)) return false;
+    if (!STORE) return true;
+    const double pre = th[0] / (UGKS_PI * th[3]), iT = 1.0 / th[3];
+    int pos = 1;
+    UGKS_VLOOP(k, NT, D.nvel) {
+        double q = q_of(th, pre, iT, D.u[k], D.v[k]);
+        qh[k] = q;
+        pos &= (q > 0.0 && isfinite(q)) ? 1 : 0;
+    }
+    return __syncthreads_and(pos) != 0;
+}
+
+// dq/dtheta_a of one velocity, h and b parts (i0 = 1/rho, iT = 1/T)
+__device__ inline void qtheta(int ck, const double th[4], double i0, double iT, double u, double v, double q,
+                             double qt_h[4], double qt_b[4]) {
+    double T = th[3];
+    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
+    double qb = 0.5 * (double)ck * T * q;
+    double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
+    qt_h[0] = q * i0; qt_h[1] = q * l1; qt_h[2] = q * l2; qt_h[3] = q * l3;
+    qt_b[0] = qb * i0; qt_b[1] = qb * l1; qt_b[2] = qb * l2; qt_b[3] = qb * (l3 + iT);
+}
+
+// S2 primitive (theta_gas = p/rho)
+__device__ inline bool s2_prim(int ck, const double w[4], double* rho, double* ux, double* vy, double* thg) {
+    *rho = w[0];
+    if (!(w[0] > 0.0)) return false;
+    *ux = w[1] / w[0];
+    *vy = w[2] / w[0];
+    double eint = w[3] - 0.5 * w[0] * ((*ux) * (*ux) + (*vy) * (*vy));
+    if (!(eint > 0.0)) return false;
+    double p = (2.0 / (double)(ck + 2)) * eint;
+    *thg = p / w[0];
+    return isfinite(*thg) && *thg > 0.0;
+}
+
+// ===========================================================================
+// multiscale diffuse wall (physical gas): one workgroup per wall face
+// ===========================================================================
+__device__ inline void micro_slope(int ck, const double prim[4], const double sw[4], double a[4]) {
+    a[3] = 4.0 * prim[3] * prim[3] / ((ck + 2) * prim[0]) *
+           (2.0 * sw[3] - 2.0 * prim[1] * sw[1] - 2.0 * prim[2] * sw[2] +
+            sw[0] * (prim[1] * prim[1] + prim[2] * prim[2] - 0.5 * (ck + 2) / prim[3]));
+    a[2] = 2.0 * prim[3] / prim[0] * (sw[2] - prim[2] * sw[0]) - prim[2] * a[3];
+    a[1] = 2.0 * prim[3] / prim[0] * (sw[1] - prim[1] * sw[0]) - prim[1] * a[3];
+    a[0] = sw[0] / prim[0] - prim[1] * a[1] - prim[2] * a[2] -
+           0.5 * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (ck + 2) / prim[3]) * a[3];
+}
+
+__device__ inline void half_moments(int ck, const double prim[4], double mu[7], double mv[6], double mxi[3]) {
+    double mul[7], mur[7];
+    mul[0] = 0.5 * erfc(-sqrt(prim[3]) * prim[1]);
+    mul[1] = prim[1] * mul[0] + 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
+    mur[0] = 0.5 * erfc(sqrt(prim[3]) * prim[1]);
+    mur[1] = prim[1] * mur[0] - 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
+    for (int i = 2; i <= 6; ++i) {
+        mul[i] = prim[1] * mul[i - 1] + 0.5 * (i - 1) * mul[i - 2] / prim[3];
+        mur[i] = prim[1] * mur[i - 1] + 0.5 * (i - 1) * mur[i - 2] / prim[3];
+    }
+    for (int i = 0; i <= 6; ++i) mu[i] = mul[i] + mur[i];
+    mv[0] = 1.0;
+    mv[1] = prim[2];
+    for (int i = 2; i <= 5; ++i) mv[i] = prim[2] * mv[i - 1] + 0.5 * (i - 1) * mv[i - 2] / prim[3];
+    mxi[0] = 1.0;
+    mxi[1] = 0.5 * ck / prim[3];
+    mxi[2] = ck * (ck + 2.0) / (4.0 * prim[3] * prim[3]);
+}
+
+__device__ inline void m_uvxi(const double mu[7], const double mv[6], const double mxi[3], int al, int be, int de,
+                              double o[4]) {
+    o[0] = mu[al] * mv[be] * mxi[de / 2];
+    o[1] = mu[al + 1] * mv[be] * mxi[de / 2];
+    o[2] = mu[al] * mv[be + 1] * mxi[de / 2];
+    o[3] = 0.5 * (mu[al + 2] * mv[be] * mxi[de / 2] + mu[al] * mv[be + 2] * mxi[de / 2] +
+                  mu[al] * mv[be] * mxi[(de + 2) / 2]);
+}
+
+__device__ inline void m_auvxi(const double a[4], const double mu[7], const double mv[6], const double mxi[3], int al,
+                               int be, double o[4]) {
+    double t0[4], t1[4], t2[4], t3[4], t4[4], t5[4];
+    m_uvxi(mu, mv, mxi, al, be, 0, t0);
+    m_uvxi(mu, mv, mxi, al + 1, be, 0, t1);
+    m_uvxi(mu, mv, mxi, al, be + 1, 0, t2);
+    m_uvxi(mu, mv, mxi, al + 2, be, 0, t3);
+    m_uvxi(mu, mv, mxi, al, be + 2, 0, t4);
+    m_uvxi(mu, mv, mxi, al, be, 2, t5);
+    for (int c = 0; c < 4; ++c)
+        o[c] = a[0] * t0[c] + a[1] * t1[c] + a[2] * t2[c] + 0.5 * a[3] * t3[c] + 0.5 * a[3] * t4[c] +
+               0.5 * a[3] * t5[c];
+}
+
+__device__ inline void wall_time_coeffs(double dt, double tau, double mt[5]) {
+    double x = dt / tau;
+    if (x < 0.1) {
+        double t = 1.0;
+        for (int a = 0; a < 5; ++a) mt[a] = 0.0;
+        for (int n = 1; n <= 24; ++n) {
+            t = t * (-x) / (double)n;
+            mt[3] = mt[3] - tau * t;
+            if (n >= 2) {
+                mt[0] = mt[0] + tau * t;
+                mt[4] = mt[4] + tau * tau * (double)(n - 1) * t;
+            }
+            if (n >= 3) {
+                mt[1] = mt[1] + tau * tau * (double)(n - 2) * t;
+                mt[2] = mt[2] - tau * tau * t;
+            }
+        }
+    } else {
+        double ex = exp(-x);
+        mt[3] = tau * (1.0 - ex);
+        mt[4] = -tau * dt * ex + tau * mt[3];
+        mt[0] = dt - mt[3];
+        mt[1] = -tau * mt[0] + mt[4];
+        mt[2] = 0.5 * dt * dt - tau * mt[0];
+    }
+}
+
+// incident-side wall distribution of one velocity at one Gauss node
+__device__ inline void wall_gin(const double mt[5], const double an[4], const double at[4], const double att[4],
+                                const double mxi[3], double coeff, double lambda, double ux, int ck, bool shakhov,
+                                const double red[6], double u, double v, double mh, double mb, double hin, double hn,
+                                double ht, double bin, double bn, double bt, double* gh, double* gb) {
+    double vn = v, vt = -u, v2 = vn * vn + vt * vt;
+    double corrh = 0.0, corrb = 0.0;
+    if (shakhov) {
+        double vtl = -u + ux, v2l = vn * vn + vtl * vtl;
+        double qdot = vn * red[4] + vtl * red[5], shh = 2.0 * lambda * v2l - 4.0;
+        corrh = coeff * qdot * shh * mh;
+        corrb = coeff * qdot * (shh + 2.0 / (double)ck) * mb;
+    }
+    double Han = (an[0] + an[1] * vn + an[2] * vt) * mh + 0.5 * an[3] * (v2 * mh + mb);
+    double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh + 0.5 * at[3] * (v2 * mh + mb);
+    double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh + 0.5 * att[3] * (v2 * mh + mb);
+    double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb + 0.5 * an[3] * (v2 * mb + mxi[2] * mh);
+    double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb + 0.5 * at[3] * (v2 * mb + mxi[2] * mh);
+    double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb + 0.5 * att[3] * (v2 * mb + mxi[2] * mh);
+    double g_h = mt[0] * mh + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin - mt[4] * (vn * hn + u * ht);
+    double g_b = mt[0] * mb + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin - mt[4] * (vn * bn + u * bt);
+    *gh = g_h + mt[0] * corrh;
+    *gb = g_b + mt[0] * corrb;
+}
+
+// heat flux contribution of one velocity relative to the bulk velocity (ux, vy)
+__device__ inline void heat_add(double q[2], double w, double cx, double cy, double h, double b) {
+    const double c2 = cx * cx + cy * cy;
+    q[0] += 0.5 * w * cx * (c2 * h + b);
+    q[1] += 0.5 * w * cy * (c2 * h + b);
+}
+

@@ -10,8 +10,7 @@
 //                                                   src/mod_S2NativeAdapter.f90:735-759
 //   UpdateImplicitResidualDiagnostics               src/mod_ImplicitDiagnostics.f90:223-322
 //   TimeStep                                         src/mod_Solve.f90:373-421
-#include "ugks_recon.h"
-#include "ugks_state.h"
+#include "ugks_node.h"
 #include "ugks_prof.h"
 
 __constant__ Consts KC;
@@ -89,81 +88,6 @@ __global__ void __launch_bounds__(256) k_wrecon(const Dev* __restrict__ DS, int 
         for (int b = 0; b < 3; ++b) ov[a * 3 + b] = vol[a][b];
 }
 
-// ===========================================================================
-// micro reconstruction: one thread per (cell, velocity), h and b together
-// ===========================================================================
-struct SideOut {
-    double d[3][3];   // node x {value, dn, dt}
-};
-
-// Second stage of one field at cell (i, j): the FirstLayer gathered from the
-// lines of the column stencil (rows rb..rb+4 at column i) and of the row
-// stencil (columns i-2..i+2 of row j); t = 2k + field within the cell.
-__device__ inline void recon_field(const double* lines, int nx, int nvel, int i, int j, int rb, bool near, int sh,
-                                   int t, int xs, int ys, int wsd, double fval[4][3], double vol[3][3], SideOut& xo,
-                                   SideOut& yo, SideOut& wo) {
-    const size_t row = 2 * (size_t)nvel;
-    FirstLayer L;
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        const double* p = lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
-        L.xv[r][0] = p[0]; L.xv[r][1] = p[row]; L.xd[r][0] = p[2 * row]; L.xd[r][1] = p[3 * row];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) L.rows[r][a] = p[(4 + a) * row];
-    }
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        int col = i + s - 2;
-        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
-        const double* p = lines + (size_t)(j * nx + col) * LINE_ROWS * row + t;
-        L.yv[s][0] = p[7 * row]; L.yv[s][1] = p[8 * row]; L.yd[s][0] = p[9 * row]; L.yd[s][1] = p[10 * row];
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        double d[3][3];
-        side_data(KC, L, near, sh, s, d);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            fval[s][a] = d[a][0];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (s == xs) xo.d[a][q] = d[a][q];
-                if (s == ys) yo.d[a][q] = d[a][q];
-                if (s == wsd) wo.d[a][q] = d[a][q];
-            }
-        }
-    }
-    vol_data(KC, L, near, sh, vol);
-}
-
-__device__ inline double theta_of(double m, double fval[4][3], double vol[3][3], double theta) {
-    const double keep = 1.0 - 64.0 * UGKS_DBL_EPS;
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (fval[s][a] < 0.0) theta = fmin(theta, keep * m / (m - fval[s][a]));
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-            if (vol[a][b] < 0.0) theta = fmin(theta, keep * m / (m - vol[a][b]));
-    return theta;
-}
-
-// write one limited side of one field (rows off..off+2 of each node) into a
-// face record; isn, ist: reciprocals of the normal and tangential cell sizes
-__device__ inline void store_side(double* rec, size_t nv, int off, double theta, double m, const SideOut& so,
-                                  double isn, double ist) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double* p = rec + (size_t)(a * 6 + off) * nv;
-        p[0] = m + theta * (so.d[a][0] - m);
-        p[nv] = theta * so.d[a][1] * isn;
-        p[2 * nv] = theta * so.d[a][2] * ist;
-    }
-}
-
 // First stage: the x and y lines of every (cell, velocity, field); two
 // lanes per (cell, velocity), the even lane for h and the odd lane for b.
 __global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS) {
@@ -220,7 +144,7 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     double fv[4][3], vol[3][3];
     SideOut xo, yo, wo;
     const double m = F[(size_t)c * nv + k];
-    recon_field(D.lines, nx, nvel, i, j, rb, near, sh, t, xs, ys, wsd, fv,
+    recon_field(KC, D.lines, nx, nvel, i, j, rb, near, sh, t, xs, ys, wsd, fv,
                 vol, xo, yo, wo);
     double theta = theta_of(m, fv, vol, 1.0);
     theta = fmin(theta, lane_xor1(theta));
@@ -243,154 +167,6 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
 }
 
 // ===========================================================================
-// velocity-parallel closure machinery
-// ===========================================================================
-// Velocity loops: thread t of an NT-thread workgroup owns velocities t, t+NT,
-// t+2NT, ...  Per-velocity inputs (velocity table, records, states) are read
-// from memory in every pass and stay in the caches, so nothing per velocity is
-// held in registers across a workgroup reduction.  Per-velocity values shared
-// by several passes live in LDS slots indexed by velocity; a slot is only ever
-// touched by the thread that owns the velocity, so the slots need no barriers.
-#define UGKS_VLOOP(k, NT, n) _Pragma("unroll 1") for (int k = threadIdx.x; k < (n); k += (NT))
-
-// q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q: the moments of q and of
-// dq/dtheta_a (Jacobian columns), red[0..3] moments, red[4+4a..] column a.
-template <int NT>
-__device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds) {
-#pragma unroll
-    for (int a = 0; a < 20; ++a) red[a] = 0.0;
-    const double r0 = th[0], T = uni(th[3]), t1 = uni(th[1]), t2 = uni(th[2]);
-    const double pre = uni(r0 / (UGKS_PI * T));
-    const double i0 = uni(1.0 / r0), iT = uni(1.0 / T);
-    const int nvel = D.nvel;
-    UGKS_VLOOP(k, NT, nvel) {
-        const double u = D.u[k], v = D.v[k], w = D.w[k];
-        double du = u - t1, dv = v - t2, c2 = du * du + dv * dv;
-        double q = pre * exp(-c2 * iT);
-        double qb = 0.5 * (double)ck * T * q;
-        double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
-        double lb3 = l3 + iT;
-        mom_add(red + 0, w, u, v, q, qb);
-        mom_add(red + 4, w, u, v, q * i0, qb * i0);
-        mom_add(red + 8, w, u, v, q * l1, qb * l1);
-        mom_add(red + 12, w, u, v, q * l2, qb * l2);
-        mom_add(red + 16, w, u, v, q * l3, qb * lb3);
-    }
-    block_sum<20>(red, lds);
-}
-
-// q(theta) of one velocity (the expression q_pass uses); iT = 1/T.  Hot
-// loops multiply by reciprocals of the uniform T, rho instead of dividing.
-__device__ inline double q_of(const double th[4], double pre, double iT, double u, double v) {
-    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
-    return pre * exp(-c2 * iT);
-}
-
-__device__ inline double scaled_res(const double r[4], const double w[4]) {
-    double m = 0.0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) m = fmax(m, fabs(r[a]) / fmax(1.0, fabs(w[a])));
-    return m;
-}
-
-// Newton solve for the discrete conservative equilibrium with moments w
-// (halving line search down to 2^-20, at most 12 steps).  On success th and
-// J describe the solution and, with STORE, qh[k] (LDS slots of the owned
-// velocities) holds q(th), all positive and finite.
-// With STORE false the caller computes q itself and must check positivity.
-template <int NT, bool STORE>
-__device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], double th[4], double J[16], double* qh,
-                                     double* lds) {
-    if (!(isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && isfinite(w[3])) || !(w[0] > 0.0)) return false;
-    th[0] = w[0];
-    th[1] = w[1] / w[0];
-    th[2] = w[2] / w[0];
-    th[3] = 4.0 * (w[3] / w[0] - 0.5 * (th[1] * th[1] + th[2] * th[2])) / (double)(ck + 2);
-    if (!(th[3] > 0.0) || !isfinite(th[3])) return false;
-    double red[20], res[4];
-    q_pass<NT>(ck, D, th, red, lds);
-#pragma unroll
-    for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
-    double nrm = scaled_res(res, w);
-    for (int it = 0; it < 12; ++it) {
-        if (nrm <= UGKS_NEWTON_TOL) break;
-        double step[4], mres[4] = {-res[0], -res[1], -res[2], -res[3]};
-        if (!solve4(J, mres, step)) return false;
-        // line search: J and res are rebuilt from the accepted trial, so
-        // neither is live across the trial passes
-        double alpha = 1.0, nn = 0.0;
-        bool acc = false;
-        while (alpha >= 9.5367431640625e-07) {
-            double nt[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) nt[a] = th[a] + alpha * step[a];
-            if (nt[0] > 0.0 && nt[3] > 0.0 && isfinite(nt[0]) && isfinite(nt[1]) && isfinite(nt[2]) &&
-                isfinite(nt[3])) {
-                q_pass<NT>(ck, D, nt, red, lds);
-                bool fin = true;
-#pragma unroll
-                for (int a = 0; a < 20; ++a) fin = fin && isfinite(red[a]);
-                if (fin) {
-                    double nres[4];
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) nres[a] = red[a] - w[a];
-                    nn = scaled_res(nres, w);
-                    if (nn < nrm || nn <= UGKS_NEWTON_TOL) {
-                        acc = true;
-#pragma unroll
-                        for (int a = 0; a < 4; ++a) th[a] = nt[a];
-                    }
-                }
-                if (acc) break;
-            }
-            alpha *= 0.5;
-        }
-        if (!acc) return false;
-#pragma unroll
-        for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) res[a] = red[a] - w[a];
-        nrm = nn;
-    }
-    if (!(nrm <= UGKS_NEWTON_TOL)) return false;
-    if (!STORE) return true;
-    const double pre = th[0] / (UGKS_PI * th[3]), iT = 1.0 / th[3];
-    int pos = 1;
-    UGKS_VLOOP(k, NT, D.nvel) {
-        double q = q_of(th, pre, iT, D.u[k], D.v[k]);
-        qh[k] = q;
-        pos &= (q > 0.0 && isfinite(q)) ? 1 : 0;
-    }
-    return __syncthreads_and(pos) != 0;
-}
-
-// dq/dtheta_a of one velocity, h and b parts (i0 = 1/rho, iT = 1/T)
-__device__ inline void qtheta(int ck, const double th[4], double i0, double iT, double u, double v, double q,
-                             double qt_h[4], double qt_b[4]) {
-    double T = th[3];
-    double du = u - th[1], dv = v - th[2], c2 = du * du + dv * dv;
-    double qb = 0.5 * (double)ck * T * q;
-    double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
-    qt_h[0] = q * i0; qt_h[1] = q * l1; qt_h[2] = q * l2; qt_h[3] = q * l3;
-    qt_b[0] = qb * i0; qt_b[1] = qb * l1; qt_b[2] = qb * l2; qt_b[3] = qb * (l3 + iT);
-}
-
-// S2 primitive (theta_gas = p/rho)
-__device__ inline bool s2_prim(int ck, const double w[4], double* rho, double* ux, double* vy, double* thg) {
-    *rho = w[0];
-    if (!(w[0] > 0.0)) return false;
-    *ux = w[1] / w[0];
-    *vy = w[2] / w[0];
-    double eint = w[3] - 0.5 * w[0] * ((*ux) * (*ux) + (*vy) * (*vy));
-    if (!(eint > 0.0)) return false;
-    double p = (2.0 / (double)(ck + 2)) * eint;
-    *thg = p / w[0];
-    return isfinite(*thg) && *thg > 0.0;
-}
-
-// ===========================================================================
 // UGKS interior face flux: one workgroup per face, three Gauss nodes
 // ===========================================================================
 // One Gauss node.  rec: the node's upwind record, rows {h, h_n, h_t, b, b_n,
@@ -399,7 +175,7 @@ __device__ inline bool s2_prim(int ck, const double w[4], double* rho, double* u
 template <int NT>
 __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const double* rec, const double* sl_p,
                                      const double* sr_p, double ln, double lt, double gwa, bool first, double* oh,
-                                     double* ob, double fw_acc[4], double* qh, double* lds) {
+                                     double* ob, double fw_acc[4], double* qh, double* lds, double* tape) {
     const int ck = D.gas.ck, nvel = D.nvel;
     const size_t nv = (size_t)nvel;
     double w0[4] = {0, 0, 0, 0};
@@ -472,6 +248,14 @@ __device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const do
     block_sum<4>(wt4, lds);
     double wtime[4] = {-wt4[0], -wt4[1], -wt4[2], -wt4[3]}, ttm[4];
     if (!solve4(J, wtime, ttm)) return ST_SOLVE;
+    if (tape && threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) { tape[a] = w0[a]; tape[4 + a] = th[a]; tape[34 + a] = wt4[a]; }
+#pragma unroll
+        for (int a = 0; a < 16; ++a) tape[8 + a] = J[a];
+#pragma unroll
+        for (int a = 0; a < 10; ++a) tape[24 + a] = red[a];
+    }
     uni<4>(ttm);
     // variable-tau time integrals
     double w0c[4] = {w0[0], w0[1], w0[2], w0[3]};
@@ -566,7 +350,7 @@ __global__ void __launch_bounds__(NT) UGKS_CLOSURE_ATTR k_face_flux(const Dev* _
         const double* pl = D.ws + (size_t)(cl * 4) * 24 + (sl * 3 + a) * 2;
         const double* pr = D.ws + (size_t)(cr * 4) * 24 + (sr * 3 + a) * 2;
         st = ugks_node<NT>(D, axis, dt, rec + (size_t)(a * 6) * nvel, pl, pr, ln, lt, KC.gw[a], a == 0, oh, ob,
-                           fw_acc, qh, lds);
+                           fw_acc, qh, lds, D.ftape ? D.ftape + (size_t)(f * 3 + a) * FT_LEN : nullptr);
     }
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_FACE);
@@ -574,113 +358,6 @@ __global__ void __launch_bounds__(NT) UGKS_CLOSURE_ATTR k_face_flux(const Dev* _
     }
     if (threadIdx.x == 0)
         for (int m = 0; m < 4; ++m) ow[m] = fw_acc[m];
-}
-
-// ===========================================================================
-// multiscale diffuse wall (physical gas): one workgroup per wall face
-// ===========================================================================
-__device__ inline void micro_slope(int ck, const double prim[4], const double sw[4], double a[4]) {
-    a[3] = 4.0 * prim[3] * prim[3] / ((ck + 2) * prim[0]) *
-           (2.0 * sw[3] - 2.0 * prim[1] * sw[1] - 2.0 * prim[2] * sw[2] +
-            sw[0] * (prim[1] * prim[1] + prim[2] * prim[2] - 0.5 * (ck + 2) / prim[3]));
-    a[2] = 2.0 * prim[3] / prim[0] * (sw[2] - prim[2] * sw[0]) - prim[2] * a[3];
-    a[1] = 2.0 * prim[3] / prim[0] * (sw[1] - prim[1] * sw[0]) - prim[1] * a[3];
-    a[0] = sw[0] / prim[0] - prim[1] * a[1] - prim[2] * a[2] -
-           0.5 * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (ck + 2) / prim[3]) * a[3];
-}
-
-__device__ inline void half_moments(int ck, const double prim[4], double mu[7], double mv[6], double mxi[3]) {
-    double mul[7], mur[7];
-    mul[0] = 0.5 * erfc(-sqrt(prim[3]) * prim[1]);
-    mul[1] = prim[1] * mul[0] + 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
-    mur[0] = 0.5 * erfc(sqrt(prim[3]) * prim[1]);
-    mur[1] = prim[1] * mur[0] - 0.5 * exp(-prim[3] * prim[1] * prim[1]) / sqrt(UGKS_PI * prim[3]);
-    for (int i = 2; i <= 6; ++i) {
-        mul[i] = prim[1] * mul[i - 1] + 0.5 * (i - 1) * mul[i - 2] / prim[3];
-        mur[i] = prim[1] * mur[i - 1] + 0.5 * (i - 1) * mur[i - 2] / prim[3];
-    }
-    for (int i = 0; i <= 6; ++i) mu[i] = mul[i] + mur[i];
-    mv[0] = 1.0;
-    mv[1] = prim[2];
-    for (int i = 2; i <= 5; ++i) mv[i] = prim[2] * mv[i - 1] + 0.5 * (i - 1) * mv[i - 2] / prim[3];
-    mxi[0] = 1.0;
-    mxi[1] = 0.5 * ck / prim[3];
-    mxi[2] = ck * (ck + 2.0) / (4.0 * prim[3] * prim[3]);
-}
-
-__device__ inline void m_uvxi(const double mu[7], const double mv[6], const double mxi[3], int al, int be, int de,
-                              double o[4]) {
-    o[0] = mu[al] * mv[be] * mxi[de / 2];
-    o[1] = mu[al + 1] * mv[be] * mxi[de / 2];
-    o[2] = mu[al] * mv[be + 1] * mxi[de / 2];
-    o[3] = 0.5 * (mu[al + 2] * mv[be] * mxi[de / 2] + mu[al] * mv[be + 2] * mxi[de / 2] +
-                  mu[al] * mv[be] * mxi[(de + 2) / 2]);
-}
-
-__device__ inline void m_auvxi(const double a[4], const double mu[7], const double mv[6], const double mxi[3], int al,
-                               int be, double o[4]) {
-    double t0[4], t1[4], t2[4], t3[4], t4[4], t5[4];
-    m_uvxi(mu, mv, mxi, al, be, 0, t0);
-    m_uvxi(mu, mv, mxi, al + 1, be, 0, t1);
-    m_uvxi(mu, mv, mxi, al, be + 1, 0, t2);
-    m_uvxi(mu, mv, mxi, al + 2, be, 0, t3);
-    m_uvxi(mu, mv, mxi, al, be + 2, 0, t4);
-    m_uvxi(mu, mv, mxi, al, be, 2, t5);
-    for (int c = 0; c < 4; ++c)
-        o[c] = a[0] * t0[c] + a[1] * t1[c] + a[2] * t2[c] + 0.5 * a[3] * t3[c] + 0.5 * a[3] * t4[c] +
-               0.5 * a[3] * t5[c];
-}
-
-__device__ inline void wall_time_coeffs(double dt, double tau, double mt[5]) {
-    double x = dt / tau;
-    if (x < 0.1) {
-        double t = 1.0;
-        for (int a = 0; a < 5; ++a) mt[a] = 0.0;
-        for (int n = 1; n <= 24; ++n) {
-            t = t * (-x) / (double)n;
-            mt[3] = mt[3] - tau * t;
-            if (n >= 2) {
-                mt[0] = mt[0] + tau * t;
-                mt[4] = mt[4] + tau * tau * (double)(n - 1) * t;
-            }
-            if (n >= 3) {
-                mt[1] = mt[1] + tau * tau * (double)(n - 2) * t;
-                mt[2] = mt[2] - tau * tau * t;
-            }
-        }
-    } else {
-        double ex = exp(-x);
-        mt[3] = tau * (1.0 - ex);
-        mt[4] = -tau * dt * ex + tau * mt[3];
-        mt[0] = dt - mt[3];
-        mt[1] = -tau * mt[0] + mt[4];
-        mt[2] = 0.5 * dt * dt - tau * mt[0];
-    }
-}
-
-// incident-side wall distribution of one velocity at one Gauss node
-__device__ inline void wall_gin(const double mt[5], const double an[4], const double at[4], const double att[4],
-                                const double mxi[3], double coeff, double lambda, double ux, int ck, bool shakhov,
-                                const double red[6], double u, double v, double mh, double mb, double hin, double hn,
-                                double ht, double bin, double bn, double bt, double* gh, double* gb) {
-    double vn = v, vt = -u, v2 = vn * vn + vt * vt;
-    double corrh = 0.0, corrb = 0.0;
-    if (shakhov) {
-        double vtl = -u + ux, v2l = vn * vn + vtl * vtl;
-        double qdot = vn * red[4] + vtl * red[5], shh = 2.0 * lambda * v2l - 4.0;
-        corrh = coeff * qdot * shh * mh;
-        corrb = coeff * qdot * (shh + 2.0 / (double)ck) * mb;
-    }
-    double Han = (an[0] + an[1] * vn + an[2] * vt) * mh + 0.5 * an[3] * (v2 * mh + mb);
-    double Hat = (at[0] + at[1] * vn + at[2] * vt) * mh + 0.5 * at[3] * (v2 * mh + mb);
-    double Htt = (att[0] + att[1] * vn + att[2] * vt) * mh + 0.5 * att[3] * (v2 * mh + mb);
-    double Ban = (an[0] + an[1] * vn + an[2] * vt) * mb + 0.5 * an[3] * (v2 * mb + mxi[2] * mh);
-    double Bat = (at[0] + at[1] * vn + at[2] * vt) * mb + 0.5 * at[3] * (v2 * mb + mxi[2] * mh);
-    double Btt = (att[0] + att[1] * vn + att[2] * vt) * mb + 0.5 * att[3] * (v2 * mb + mxi[2] * mh);
-    double g_h = mt[0] * mh + mt[1] * (vn * Han + vt * Hat) + mt[2] * Htt + mt[3] * hin - mt[4] * (vn * hn + u * ht);
-    double g_b = mt[0] * mb + mt[1] * (vn * Ban + vt * Bat) + mt[2] * Btt + mt[3] * bin - mt[4] * (vn * bn + u * bt);
-    *gh = g_h + mt[0] * corrh;
-    *gb = g_b + mt[0] * corrb;
 }
 
 template <int NT>
@@ -717,6 +394,11 @@ __global__ void __launch_bounds__(NT) k_wall_flux(const Dev* __restrict__ DS) {
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_WALL);
         return;
+    }
+    double* tape = D.wtape ? D.wtape + (size_t)blockIdx.x * WT_LEN : nullptr;
+    if (tape && threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) tape[q] = th[q];
+        for (int q = 0; q < 16; ++q) tape[4 + q] = J[q];
     }
     const double bfac = 0.5 * (double)ck * th[3];
     double mu[7], mv[6], mxi[3];
@@ -766,6 +448,11 @@ __global__ void __launch_bounds__(NT) k_wall_flux(const Dev* __restrict__ DS) {
         }
         block_sum<2>(ir, lds);
         double inc = ir[0], refl = ir[1];
+        if (tape && threadIdx.x == 0) {
+            for (int q = 0; q < 6; ++q) tape[20 + 8 * a + q] = red[q];
+            tape[26 + 8 * a] = inc;
+            tape[27 + 8 * a] = refl;
+        }
         if (!isfinite(refl) || !(fabs(refl) > UGKS_DBL_MIN)) { st = ST_WALL; break; }
         double rhow = -inc / refl;
         if (!isfinite(rhow) || !(rhow > 0.0)) { st = ST_WALL; break; }
@@ -818,7 +505,7 @@ enum { VN_TH = 0, VN_UX = 4, VN_VY = 5, VN_LAM = 6, VN_COEFF = 7, VN_TDX = 8, VN
 // by thread 0) and the Gauss-averaged Maxwellian in the LDS slots amh, amb.
 template <int NT>
 __device__ inline unsigned volume_targets(const Dev& D, const double W[4], const double* wvc, double* node,
-                                          double* amh, double* amb, double* lds) {
+                                          double* amh, double* amb, double* lds, double* tape = nullptr) {
     const int nvel = D.nvel, ck = D.gas.ck;
     double avg[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -857,6 +544,11 @@ __device__ inline unsigned volume_targets(const Dev& D, const double W[4], const
         block_sum<8>(red, lds);
         double tdx[4], tdy[4];
         if (!solve4(J, red, tdx) || !solve4(J, red + 4, tdy)) return ST_SOLVE;
+        if (tape && threadIdx.x == 0) {
+            double* tp = tape + n * TT_LEN;
+            for (int q = 0; q < 4; ++q) { tp[q] = th[q]; tp[20 + q] = wq[q]; tp[24 + q] = tdx[q]; tp[28 + q] = tdy[q]; }
+            for (int q = 0; q < 16; ++q) tp[4 + q] = J[q];
+        }
         if (threadIdx.x == 0) {
             double* nd = node + n * VN_LEN;
             for (int q = 0; q < 4; ++q) { nd[VN_TH + q] = th[q]; nd[VN_TDX + q] = tdx[q]; nd[VN_TDY + q] = tdy[q]; }
@@ -1025,7 +717,8 @@ __global__ void __launch_bounds__(NT) k_cn_targets(const Dev* __restrict__ DS) {
     double* amh = dyn;
     double* amb = dyn + nvel;
     double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
-    unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds);
+    unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds,
+                                     D.ttape ? D.ttape + (size_t)(c * 2) * 9 * TT_LEN : nullptr);
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
@@ -1038,13 +731,6 @@ __global__ void __launch_bounds__(NT) k_cn_targets(const Dev* __restrict__ DS) {
         D.tg[2][cb0 + k] = akxh; D.tg[3][cb0 + k] = akxb;
         D.tg[4][cb0 + k] = akyh; D.tg[5][cb0 + k] = akyb;
     }
-}
-
-// heat flux contribution of one velocity relative to the bulk velocity (ux, vy)
-__device__ inline void heat_add(double q[2], double w, double cx, double cy, double h, double b) {
-    const double c2 = cx * cx + cy * cy;
-    q[0] += 0.5 * w * cx * (c2 * h + b);
-    q[1] += 0.5 * w * cy * (c2 * h + b);
 }
 
 // Crank-Nicolson Shakhov collision of one cell: new targets at Wn (written to
@@ -1061,7 +747,8 @@ __global__ void __launch_bounds__(NT) k_cn_cell(const Dev* __restrict__ DS) {
     double* amb = dyn + nvel;
     const double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
     const double Wn[4] = {D.Wn[4 * c], D.Wn[4 * c + 1], D.Wn[4 * c + 2], D.Wn[4 * c + 3]};
-    unsigned st = volume_targets<NT>(D, Wn, D.wv + (size_t)c * 36, node, amh, amb, lds);
+    unsigned st = volume_targets<NT>(D, Wn, D.wv + (size_t)c * 36, node, amh, amb, lds,
+                                     D.ttape ? D.ttape + (size_t)(c * 2 + 1) * 9 * TT_LEN : nullptr);
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
@@ -1110,6 +797,12 @@ __global__ void __launch_bounds__(NT) k_cn_cell(const Dev* __restrict__ DS) {
         return;
     }
     const double rnew0 = (rhs0 * a22 - a12 * rhs1) / det, rnew1 = (a11 * rhs1 - rhs0 * a21) / det;
+    if (D.ctape && threadIdx.x == 0) {
+        double* tp = D.ctape + (size_t)c * CT_LEN;
+        for (int q = 0; q < 4; ++q) tp[q] = kap[q];
+        tp[4] = qm[0]; tp[5] = qm[1]; tp[6] = rold[0]; tp[7] = rold[1]; tp[8] = rq[0]; tp[9] = rq[1];
+        tp[10] = rnew0; tp[11] = rnew1; tp[12] = beta_old; tp[13] = beta;
+    }
     // pass 3: hn, bn and the gate quantities
     double mx[6] = {0.0, 1.0, -1e308, -1e308, -1e308, -1e308};   // e1, scale, -min h, max h, -min b, max b
     double sm[6] = {0, 0, 0, 0, 0, 0};                          // moments of hn, bn (4), rhat (2)
@@ -1413,6 +1106,15 @@ extern "C" hipError_t ugks_launch_cn(const LaunchSet* L, int build_old, hipStrea
     case 256: return launch_cn<256>(*L, build_old != 0, s, P);
     default: return hipErrorInvalidValue;
     }
+}
+
+// macro reconstruction of the state (proposal 0) or of the proposal (1)
+extern "C" hipError_t ugks_launch_wrecon(const LaunchSet* L, int proposal, hipStream_t s, Prof* P) {
+    const Dev& D = *L->size;
+    UGKS_TIMED(P, K_WRECON, s,
+               hipLaunchKernelGGL(k_wrecon, dim3((4 * D.nx + 255) / 256, D.ny, L->n), dim3(256), 0, s, L->dev,
+                                  proposal));
+    return hipGetLastError();
 }
 
 extern "C" hipError_t ugks_launch_timestep(const LaunchSet* L, hipStream_t s, Prof* P) {

@@ -26,6 +26,17 @@ enum ScalarSlot {
     SC_COUNT = 96
 };
 
+// Tapes of the S2CN2 forward kept for the reverse step (written only when
+// the adjoint buffers exist):
+//   face node  [face][node][FT_LEN]: w0(4) theta(4) J(16, column-major)
+//              Shakhov moments mz_x(4) mz_y(4), q0(2), time-slope moments(4)
+//   wall face  [wall face][WT_LEN]: theta(4) J(16), then per node swt(4)
+//              hl(2) incidence reflection
+//   volume node [cell][set][node][TT_LEN]: theta(4) J(16) wq(4) tdx(4) tdy(4)
+//              (set 0: targets of the state, 1: of the proposal)
+//   CN cell    [cell][CT_LEN]: kappa(4) qm(2) rold(2) rq(2) rnew(2) beta_old beta
+enum { FT_LEN = 40, WT_LEN = 44, TT_LEN = 32, CT_LEN = 16, FJ_W = 12, WJ_W = 24 };
+
 // Upwind face record, rows of nvel values: for each Gauss node a (rows
 // 6a..6a+5) the limited h value, h normal and tangential derivatives (per
 // unit length), b value, b derivatives.
@@ -74,6 +85,25 @@ struct Dev {
     // S2CN2: volume targets {mh, mb, kxh, kxb, kyh, kyb} of the state and of the proposal
     double *tg[6], *tgn[6];
     double *cnpart;       // [ncell][8] per-cell CN diagnostics
+    // S2CN2 forward tapes (null unless the reverse step is in use)
+    double *ftape, *wtape, *ttape, *ctape;
+    // reverse step: adjoints of the proposal (inputs), of the streamed h, b,
+    // of both target sets, of the face records, of the macro reconstruction
+    // outputs, 5x5 block gradients, partial adjoints of W and Wn, parameter
+    // partials [slot][4] = {mu_ref, Pr, bottom wall speed, top wall speed}
+    double *a_lamW, *a_lamh, *a_lamb;
+    double *a_hs, *a_bs;
+    double *a_tg[6], *a_tgn[6];
+    double *a_xf, *a_yf, *a_wb, *a_wt;
+    double *a_ws, *a_wv;
+    double *a_blk, *a_blkw;
+    double *a_W, *a_Wn, *a_Wwall, *a_gwall;
+    double *a_h, *a_b, *a_Wout;
+    double *a_par, *a_sum;
+    // per-node Jacobians of the scalar chains: row 0 the values, row 1+i the
+    // derivatives along input i; face nodes [face][node][7][FJ_W], wall nodes
+    // [wall face][node][14][WJ_W]
+    double *a_fjac, *a_wjac;
     double *scal;         // ScalarSlot
     unsigned *status;
 };
