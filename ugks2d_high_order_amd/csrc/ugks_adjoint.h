@@ -508,6 +508,75 @@ __device__ inline void zero_layer(FirstLayer& L) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// table forms: the candidate polynomials' values and derivatives at the
+// points p (0: -1/2, 1: +1/2, 2-4: Gauss nodes) as weights on the samples
+// ---------------------------------------------------------------------------
+// reverse of one WENO5-AO line at NP table points: gq += d/dq of
+// sum_p (gv[p] value_p + gd[p] derivative_p)
+template <int NP>
+__device__ inline void weno_pts_adj(const Consts& K, const double q[5], const int (&pts)[NP], const double (&gv)[NP],
+                                    const double (&gd)[NP], double gq[5]) {
+    double fw[4], gfw[4];
+    weno_weights(q, fw);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double A[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int t = 0; t < 5; ++t) A[t] += gv[p] * K.pv[pts[p]][c][t] + gd[p] * K.pd[pts[p]][c][t];
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) { s += A[t] * q[t]; gq[t] += fw[c] * A[t]; }
+        gfw[c] = s;
+    }
+    weno_weights_adj(q, gfw, gq);
+}
+
+// reverse of companion3 (g[a] = {value, companion, derivative} adjoints)
+__device__ inline void comp_adj(const Consts& K, const double q[5], const double cmp[5], const double g[3][3],
+                                double gq[5], double gc[5]) {
+    double fw[4], gfw[4];
+    weno_weights(q, fw);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double A[5] = {0, 0, 0, 0, 0}, B[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                A[t] += g[a][0] * K.pv[2 + a][c][t] + g[a][2] * K.pd[2 + a][c][t];
+                B[t] += g[a][1] * K.pv[2 + a][c][t];
+            }
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            s += A[t] * q[t] + B[t] * cmp[t];
+            gq[t] += fw[c] * A[t];
+            gc[t] += fw[c] * B[t];
+        }
+        gfw[c] = s;
+    }
+    weno_weights_adj(q, gfw, gq);
+}
+
+// reverse of the near-wall x-side data (wall quartics of the samples and of
+// the companion samples at the Gauss nodes)
+__device__ inline void comp_wall_adj(const Consts& K, int sh, const double g[3][3], double gq[5], double gc[5]) {
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            a0 += g[a][0] * K.wv[sh][2 + a][t] + g[a][2] * K.wd[sh][2 + a][t];
+            a1 += g[a][1] * K.wv[sh][2 + a][t];
+        }
+        gq[t] += a0;
+        gc[t] += a1;
+    }
+}
+
 // positivity factor of one field with the candidate that sets it: arg
 // s*3+a for a face node value, 12+a*3+b for a volume node, -1 when none
 __device__ inline double theta_arg(double m, const double fval[4][3], const double vol[3][3], int* arg) {

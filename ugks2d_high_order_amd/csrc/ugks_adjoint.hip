@@ -933,6 +933,11 @@ __global__ void __launch_bounds__(NT) k_adj_wall(const Dev* __restrict__ DS) {
 // ---------------------------------------------------------------------------
 // limiter and reconstruction of h and b: two lanes per (cell, velocity)
 // ---------------------------------------------------------------------------
+// Main part: the positivity factor theta (from the forward tape) held fixed.
+// Each cell supplied three face records at most (its upwind x side, its
+// upwind y side when that face is interior, its wall side in a wall row);
+// only those sides are reversed through the second stage, then the x and y
+// lines of the 5x5 block through the first stage.
 __global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -940,7 +945,132 @@ __global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
     const int c = blockIdx.y;
     if (c >= D.ncell) return;
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
-    const size_t nv = (size_t)nvel;
+    const size_t nv = (size_t)nvel, row = 2 * nv;
+    const int i = c % nx, j = c / nx;
+    int rb, sh;
+    bool near;
+    stencil_rows(j, ny, &rb, &near, &sh);
+    const double uk = D.u[k], vk = D.v[k];
+    const int xs = uk >= 0.0 ? 1 : 0;
+    const int ys = vk >= 0.0 ? 3 : 2;
+    const double* F = fld == 0 ? D.h : D.b;
+    const double theta = D.lim[(size_t)c * nv + k];
+    const double idx = 1.0 / D.dx, idy = 1.0 / D.dy;
+    const int off = 3 * fld;
+    int cols[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        int col = i + q - 2;
+        cols[q] = col < 0 ? col + nx : (col >= nx ? col - nx : col);
+    }
+    double gb[5][5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) gb[r][q] = 0.0;
+    double gm = 0.0;
+    // x side: second stage over the column stencil's x lines, then the x lines
+    {
+        const double* gx = D.a_xf + (size_t)(xs == 1 ? c : j * nx + (i == 0 ? nx - 1 : i - 1)) * REC_ROWS * nv + k;
+        double g[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double g0 = gx[(a * 6 + off) * nv];
+            g[a][0] = theta * g0;
+            g[a][1] = theta * idx * gx[(a * 6 + off + 1) * nv];
+            g[a][2] = theta * idy * gx[(a * 6 + off + 2) * nv];
+            gm += (1.0 - theta) * g0;
+        }
+        double q[5], cm[5], gq[5] = {0, 0, 0, 0, 0}, gc[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const double* p = D.lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
+            q[r] = p[xs * row];
+            cm[r] = p[(2 + xs) * row];
+        }
+        if (near) comp_wall_adj(KA, sh, g, gq, gc);
+        else comp_adj(KA, q, cm, g, gq, gc);
+        const int pts[1] = {xs};
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            double q5[5];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) q5[s] = F[(size_t)((rb + r) * nx + cols[s]) * nv + k];
+            const double gv[1] = {gq[r]}, gd[1] = {gc[r]};
+            weno_pts_adj<1>(KA, q5, pts, gv, gd, gb[r]);
+        }
+    }
+    // y sides: the interior y face this cell is upwind of, and the wall side
+#pragma unroll
+    for (int slot = 0; slot < 2; ++slot) {
+        const double* gy = nullptr;
+        int sy = ys;
+        if (slot == 0) {
+            if (ys == 3 && j < ny - 1) gy = D.a_yf + (size_t)c * REC_ROWS * nv + k;
+            if (ys == 2 && j > 0) gy = D.a_yf + (size_t)(c - nx) * REC_ROWS * nv + k;
+        } else {
+            if (j == 0) { gy = D.a_wb + (size_t)i * REC_ROWS * nv + k; sy = 2; }
+            if (j == ny - 1) { gy = D.a_wt + (size_t)i * REC_ROWS * nv + k; sy = 3; }
+        }
+        if (!gy) continue;
+        double g[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double g0 = gy[(a * 6 + off) * nv];
+            g[a][0] = theta * g0;
+            g[a][1] = theta * idy * gy[(a * 6 + off + 1) * nv];
+            g[a][2] = theta * idx * gy[(a * 6 + off + 2) * nv];
+            gm += (1.0 - theta) * g0;
+        }
+        double q[5], cm[5], gq[5] = {0, 0, 0, 0, 0}, gc[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            const double* p = D.lines + (size_t)(j * nx + cols[s]) * LINE_ROWS * row + t;
+            q[s] = p[(7 + sy - 2) * row];
+            cm[s] = p[(9 + sy - 2) * row];
+        }
+        comp_adj(KA, q, cm, g, gq, gc);
+        const int pp = sy - 2;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            double q5[5], g5[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 5; ++r) q5[r] = F[(size_t)((rb + r) * nx + cols[s]) * nv + k];
+            if (near) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) g5[r] = gq[s] * KA.wv[sh][pp][r] + gc[s] * KA.wd[sh][pp][r];
+            } else {
+                const int pts[1] = {pp};
+                const double gv[1] = {gq[s]}, gd[1] = {gc[s]};
+                weno_pts_adj<1>(KA, q5, pts, gv, gd, g5);
+            }
+#pragma unroll
+            for (int r = 0; r < 5; ++r) gb[r][s] += g5[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+        if (r == j - rb) gb[r][2] += gm;
+    double* o = D.a_blk + (size_t)c * 25 * row + t;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) o[(size_t)(r * 5 + q) * row] = gb[r][q];
+}
+
+// Limiter part, for the (cell, velocity) pairs whose positivity factor is
+// below 1: theta = keep m / (m - f) of the candidate f that sets it; its
+// derivative is added to the block adjoints of the main part.
+__global__ void __launch_bounds__(128) k_adj_limiter(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t >> 1, fld = t & 1;
+    const int c = blockIdx.y;
+    if (c >= D.ncell) return;
+    const size_t nv = (size_t)D.nvel, row = 2 * nv;
+    const double theta = D.lim[(size_t)c * nv + k];
+    if (!(theta < 1.0) || !(theta > 0.0)) return;   // pair-uniform: both lanes leave together
+    const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const int i = c % nx, j = c / nx;
     int rb, sh;
     bool near;
@@ -957,14 +1087,8 @@ __global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
     int arg;
     const double th_lane = theta_arg(m, fv, vol, &arg);
     const double th_part = lane_xor1(th_lane);
-    const double m_other = lane_xor1(m);
-    const bool zeroed = m <= 0.0 || m_other <= 0.0;
-    double theta = fmin(th_lane, th_part);
-    if (zeroed) theta = 0.0;
-    theta = fmax(0.0, fmin(1.0, theta));
     const double idx = 1.0 / D.dx, idy = 1.0 / D.dy;
     const int off = 3 * fld;
-    // record adjoints of the sides this cell supplied
     const double* gx = D.a_xf + (size_t)(xs == 1 ? c : j * nx + (i == 0 ? nx - 1 : i - 1)) * REC_ROWS * nv + k;
     const double* gy = nullptr;
     if (ys == 3 && j < ny - 1) gy = D.a_yf + (size_t)c * REC_ROWS * nv + k;
@@ -972,89 +1096,48 @@ __global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
     const double* gw = nullptr;
     if (j == 0) gw = D.a_wb + (size_t)i * REC_ROWS * nv + k;
     if (j == ny - 1) gw = D.a_wt + (size_t)i * REC_ROWS * nv + k;
-    double gside[4][3][3];
+    double gth = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        gth += (xo.d[a][0] - m) * gx[(a * 6 + off) * nv] + xo.d[a][1] * idx * gx[(a * 6 + off + 1) * nv] +
+               xo.d[a][2] * idy * gx[(a * 6 + off + 2) * nv];
+        if (gy)
+            gth += (yo.d[a][0] - m) * gy[(a * 6 + off) * nv] + yo.d[a][1] * idy * gy[(a * 6 + off + 1) * nv] +
+                   yo.d[a][2] * idx * gy[(a * 6 + off + 2) * nv];
+        if (gw)
+            gth += (wo.d[a][0] - m) * gw[(a * 6 + off) * nv] + wo.d[a][1] * idy * gw[(a * 6 + off + 1) * nv] +
+                   wo.d[a][2] * idx * gw[(a * 6 + off + 2) * nv];
+    }
+    gth += lane_xor1(gth);
+    const bool mine = th_lane < th_part || (th_lane == th_part && fld == 0);
+    if (!mine || arg < 0) return;
+    double f = 0.0;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int e = 0; e < 3; ++e) gside[s][a][e] = 0.0;
-    double gm = 0.0, gth = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        // x side
-        {
-            const double g0 = gx[(a * 6 + off) * nv], g1 = gx[(a * 6 + off + 1) * nv], g2 = gx[(a * 6 + off + 2) * nv];
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-                if (s == xs) {
-                    gside[s][a][0] += theta * g0;
-                    gside[s][a][1] += theta * idx * g1;
-                    gside[s][a][2] += theta * idy * g2;
-                }
-            gm += (1.0 - theta) * g0;
-            gth += (xo.d[a][0] - m) * g0 + xo.d[a][1] * idx * g1 + xo.d[a][2] * idy * g2;
-        }
-        if (gy) {
-            const double g0 = gy[(a * 6 + off) * nv], g1 = gy[(a * 6 + off + 1) * nv], g2 = gy[(a * 6 + off + 2) * nv];
-#pragma unroll
-            for (int s = 2; s < 4; ++s)
-                if (s == ys) {
-                    gside[s][a][0] += theta * g0;
-                    gside[s][a][1] += theta * idy * g1;
-                    gside[s][a][2] += theta * idx * g2;
-                }
-            gm += (1.0 - theta) * g0;
-            gth += (yo.d[a][0] - m) * g0 + yo.d[a][1] * idy * g1 + yo.d[a][2] * idx * g2;
-        }
-        if (gw) {
-            const double g0 = gw[(a * 6 + off) * nv], g1 = gw[(a * 6 + off + 1) * nv], g2 = gw[(a * 6 + off + 2) * nv];
-#pragma unroll
-            for (int s = 2; s < 4; ++s)
-                if (s == wsd) {
-                    gside[s][a][0] += theta * g0;
-                    gside[s][a][1] += theta * idy * g1;
-                    gside[s][a][2] += theta * idx * g2;
-                }
-            gm += (1.0 - theta) * g0;
-            gth += (wo.d[a][0] - m) * g0 + wo.d[a][1] * idy * g1 + wo.d[a][2] * idx * g2;
-        }
-    }
-    gth += lane_xor1(gth);
-    double gvol[3][3];
+            if (arg == s * 3 + a) f = fv[s][a];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) gvol[a][b] = 0.0;
-    const bool mine = th_lane < th_part || (th_lane == th_part && fld == 0);
-    if (!zeroed && theta > 0.0 && theta < 1.0 && mine && arg >= 0) {
-        double f = 0.0;
+        for (int b = 0; b < 3; ++b)
+            if (arg == 12 + a * 3 + b) f = vol[a][b];
+    const double keep = 1.0 - 64.0 * UGKS_DBL_EPS, den = m - f;
+    const double gm = gth * keep * (-f) / (den * den);
+    const double gf = gth * keep * m / (den * den);
+    double gside[4][3][3], gvol[3][3];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (arg == s * 3 + a) f = fv[s][a];
+        for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+            for (int e = 0; e < 3; ++e) gside[s][a][e] = 0.0;
+            if (arg == s * 3 + a) gside[s][a][0] = gf;
+        }
 #pragma unroll
-            for (int b = 0; b < 3; ++b)
-                if (arg == 12 + a * 3 + b) f = vol[a][b];
-        const double keep = 1.0 - 64.0 * UGKS_DBL_EPS, den = m - f;
-        gm += gth * keep * (-f) / (den * den);
-        const double gf = gth * keep * m / (den * den);
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (arg == s * 3 + a) gside[s][a][0] += gf;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-                if (arg == 12 + a * 3 + b) gvol[a][b] += gf;
-    }
-    // second stage reversed from the first-stage lines, then the first stage
-    const size_t row = 2 * nv;
+        for (int b = 0; b < 3; ++b) gvol[a][b] = arg == 12 + a * 3 + b ? gf : 0.0;
     FirstLayer L, gL;
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
@@ -1090,7 +1173,7 @@ __global__ void __launch_bounds__(128) k_adj_recon(const Dev* __restrict__ DS) {
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-        for (int s = 0; s < 5; ++s) o[(size_t)(r * 5 + s) * row] = gblk[r][s];
+        for (int s = 0; s < 5; ++s) o[(size_t)(r * 5 + s) * row] += gblk[r][s];
 }
 
 // lam_h, lam_b: direct part + streamed part + gathered block adjoints
@@ -1162,6 +1245,8 @@ static hipError_t launch_adjoint(const LaunchSet& L, hipStream_t s, Prof* P) {
                hipLaunchKernelGGL(k_adj_wgather, dim3((4 * D.ncell + 255) / 256, 1, n), dim3(256), 0, s, L.dev, 0));
     UGKS_TIMED(P, K_ADJ_RECON, s,
                hipLaunchKernelGGL(k_adj_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
+    UGKS_TIMED(P, K_ADJ_LIMITER, s,
+               hipLaunchKernelGGL(k_adj_limiter, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
     UGKS_TIMED(P, K_ADJ_GATHER, s,
                hipLaunchKernelGGL(k_adj_gather, dim3((2 * D.nvel + 255) / 256, D.ncell, n), dim3(256), 0, s, L.dev));
     const int nslots = D.ncell + nfaces + 2 * D.nx;

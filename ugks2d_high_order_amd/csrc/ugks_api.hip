@@ -123,6 +123,33 @@ int build_consts() {
         average_map(5, off, m);
         memcpy(K.wallmap[sh], m, sizeof m);
     }
+    const double xs[5] = {-0.5, 0.5, K.gnode[0], K.gnode[1], K.gnode[2]};
+    for (int p = 0; p < 5; ++p) {
+        for (int c = 0; c < 4; ++c)
+            for (int t = 0; t < 5; ++t) {
+                double v = 0.0, d = 0.0, x = 1.0, xm = 0.0;
+                for (int e = 0; e < 5; ++e) {
+                    v += K.cmap[c][t][e] * x;
+                    d += e * K.cmap[c][t][e] * xm;
+                    xm = e == 0 ? 1.0 : xm * xs[p];
+                    x *= xs[p];
+                }
+                K.pv[p][c][t] = v;
+                K.pd[p][c][t] = d;
+            }
+        for (int sh = 0; sh < 5; ++sh)
+            for (int t = 0; t < 5; ++t) {
+                double v = 0.0, d = 0.0, x = 1.0, xm = 0.0;
+                for (int e = 0; e < 5; ++e) {
+                    v += K.wallmap[sh][t][e] * x;
+                    d += e * K.wallmap[sh][t][e] * xm;
+                    xm = e == 0 ? 1.0 : xm * xs[p];
+                    x *= xs[p];
+                }
+                K.wv[sh][p][t] = v;
+                K.wd[sh][p][t] = d;
+            }
+    }
     HIPCHK(ugks_set_consts(&K));
     HIPCHK(ugks_set_consts_adjoint(&K));
     return E_OK;
@@ -485,7 +512,7 @@ static int adj_buffers(Instance* I) {
         {&D.a_blk, nc * 25 * 2 * nv}, {&D.a_blkw, nc * 100}, {&D.a_W, 4 * nc}, {&D.a_Wn, 4 * nc},
         {&D.a_Wwall, 8 * nx}, {&D.a_gwall, 8 * nx}, {&D.a_h, cv}, {&D.a_b, cv}, {&D.a_Wout, 4 * nc},
         {&D.a_par, 4 * (nc + nfaces + 2 * nx)}, {&D.a_sum, 4}, {&D.a_fjac, nfaces * 3 * 7 * FJ_W},
-        {&D.a_wjac, 2 * nx * 3 * 14 * WJ_W}};
+        {&D.a_wjac, 2 * nx * 3 * 14 * WJ_W}, {&D.lim, cv}};
     for (auto& q : bufs)
         if ((rc = dmalloc(I, q.p, q.n))) return rc;
     for (int q = 0; q < 6; ++q)

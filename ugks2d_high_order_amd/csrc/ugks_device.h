@@ -43,6 +43,11 @@ struct Consts {
     double gw[3];
     double cmap[4][5][5];      // WENO candidate maps [candidate][sample][degree]
     double wallmap[5][5][5];   // one-sided quartic maps, samples at offsets -sh..4-sh
+    // the same maps evaluated at the points p (0: -1/2, 1: +1/2, 2-4: Gauss
+    // nodes): value pv and derivative pd of candidate c per sample t, and of
+    // the wall quartic of shift sh
+    double pv[5][4][5], pd[5][4][5];
+    double wv[5][5][5], wd[5][5][5];
 };
 
 // ---------------------------------------------------------------------------

@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     const double m_other = lane_xor1(m);
     if (m <= 0.0 || m_other <= 0.0) theta = 0.0;
     theta = fmax(0.0, fmin(1.0, theta));
+    if (D.lim && fld == 0) D.lim[(size_t)c * nv + k] = theta;
     const double idx = 1.0 / D.dx, idy = 1.0 / D.dy;
     const int off = 3 * fld;
     // x face: upwind side of this cell

@@ -104,6 +104,7 @@ struct Dev {
     // derivatives along input i; face nodes [face][node][7][FJ_W], wall nodes
     // [wall face][node][14][WJ_W]
     double *a_fjac, *a_wjac;
+    double *lim;          // [ncell][nvel] positivity factor of each (cell, velocity) (forward tape)
     double *scal;         // ScalarSlot
     unsigned *status;
 };
