@@ -129,6 +129,18 @@ int ugks_cn_adjoint_device(void* handle, const double* W, const double* h, const
                            const double* lamW, const double* lamh, const double* lamb, const double* gwall,
                            double* gW, double* gh, double* gb, double* gpar4);
 
+/* Reverse of n consecutive S2CN2 steps from the state S_0 = (W, h, b): the
+ * n forward steps run once with their buffers kept (about 70 doubles per cell
+ * and velocity per step of device memory), then the steps are reversed in
+ * the opposite order without re-running them.  lamW, lamh, lamb: adjoint of
+ * S_n; gwall: host array [n][2][nx][4], the adjoint of each step's wall
+ * fluxes.  Out: gW, gh, gb, the adjoint of S_0, and on the host gpar4 =
+ * d/d(mu_ref, Pr, bottom wall speed, top wall speed) summed over the steps.
+ * State-sized arrays are device pointers; no counterpart in the reference. */
+int ugks_cn_adjoint_segment(void* handle, const double* W, const double* h, const double* b, int n,
+                            const double* lamW, const double* lamh, const double* lamb, const double* gwall,
+                            double* gW, double* gh, double* gb, double* gpar4);
+
 /* Low-order Explicit entry (TimeIntegration_method = 'Explicit'): Van Leer
  * slopes, the analytic-moment UGKS face flux with the projected face
  * equilibrium and its Shakhov part, the multiscale diffuse walls and the

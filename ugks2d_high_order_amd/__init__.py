@@ -26,7 +26,7 @@ EXPORTS = [
     "ugks_explicit_step", "ugks_explicit_cold_start",
     "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
     "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
-    "ugks_cn_adjoint_device",
+    "ugks_cn_adjoint_device", "ugks_cn_adjoint_segment",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -80,6 +80,8 @@ def load_library(path: str = LIB_PATH):
     L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
     L.ugks_wall_fluxes.argtypes = [C.c_void_p, _D]
     L.ugks_cn_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_D] + [C.c_void_p] * 3 + [_D]
+    L.ugks_cn_adjoint_segment.argtypes = ([C.c_void_p] + [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 + [_D]
+                                          + [C.c_void_p] * 3 + [_D])
     L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.ugks_batch_create.argtypes = [C.c_int, C.POINTER(Params), _D, _D, _D, C.POINTER(C.c_void_p)]
     L.ugks_batch_destroy.argtypes = [C.c_void_p]
@@ -253,6 +255,19 @@ class UGKS:
         out = np.zeros(4)
         _check(load_library().ugks_cn_adjoint_device(self.handle, W, h, b, lamW, lamh, lamb, _ptr(g), gW, gh, gb,
                                                       _ptr(out)))
+        return out
+
+    def cn_adjoint_segment(self, W, h, b, n, lamW, lamh, lamb, gwall, gW, gh, gb):
+        """Reverse of n S2CN2 steps from the state (W, h, b) (device
+        addresses): writes the adjoint of that state into (gW, gh, gb) given
+        the adjoint of the state n steps later and the adjoints of the n
+        steps' wall fluxes gwall (host array [n, 2, nx, 4]); returns the
+        summed gradients with respect to (mu_ref, Pr, bottom wall speed, top
+        wall speed)."""
+        g = np.ascontiguousarray(gwall, dtype=np.float64).reshape(n, 2, self.case.nx, 4)
+        out = np.zeros(4)
+        _check(load_library().ugks_cn_adjoint_segment(self.handle, W, h, b, int(n), lamW, lamh, lamb, _ptr(g), gW,
+                                                       gh, gb, _ptr(out)))
         return out
 
     def run(self, n: int) -> int:

@@ -139,6 +139,21 @@ class HipStepper:
                                  *[x.data_ptr() for x in out])
         return self._shape(*out), gp
 
+    def reverse_segment(self, state, n, lam, gwalls, params):
+        """Adjoint of the state n steps before the state whose adjoint is
+        lam, running and reversing the n steps on the device in one call
+        (ugks_cn_adjoint_segment); gwalls: the n steps' wall-flux adjoints."""
+        g = self._instance(params)
+        S = self._flat(*state)
+        Lm = self._flat(*lam)
+        out = [torch.empty_like(x) for x in S]
+        gw = np.stack([np.asarray(x.detach().cpu(), dtype=np.float64) for x in gwalls])
+        if self.model.dev.type == "cuda":
+            torch.cuda.current_stream(self.model.dev).synchronize()
+        gp = g.cn_adjoint_segment(*[x.data_ptr() for x in S], n, *[x.data_ptr() for x in Lm], gw,
+                                  *[x.data_ptr() for x in out])
+        return self._shape(*out), gp
+
     def close(self):
         if self._g is not None:
             self._g.close()
@@ -204,14 +219,15 @@ def _reverse_hip(ctx, gJ):
     gU = gmu = gpr = 0.0
     for seg in reversed(range(0, K, every)):
         n = min(every, K - seg)
-        W, h, b = ctx.kept[seg // every]
-        states = stepper.run(W, h, b, params, n - 1, keep_every=1)[0] if n > 1 else [(W, h, b)]
-        for k in reversed(range(n)):
+        gws = []
+        for k in range(n):
             gw, dU = _wall_adjoint(model, ctx.fws[seg + k], params[0], gJ)
-            lam, gp = stepper.reverse(states[k], lam, gw, params)
-            gU += float(dU) - gp[2] + gp[3]
-            gmu += gp[0]
-            gpr += gp[1]
+            gws.append(gw)
+            gU += float(dU)
+        lam, gp = stepper.reverse_segment(ctx.kept[seg // every], n, lam, gws, params)
+        gU += -gp[2] + gp[3]
+        gmu += gp[0]
+        gpr += gp[1]
     f = lambda x: torch.tensor(x, dtype=torch.float64, device=model.dev)  # noqa: E731
     return lam, (f(gU), f(gmu), f(gpr))
 

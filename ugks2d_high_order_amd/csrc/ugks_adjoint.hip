@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(NT) k_adj_targets(const Dev* __restrict__ DS, 
     const double* gkxb = G[3] + cb0;
     const double* gkyh = G[4] + cb0;
     const double* gkyb = G[5] + cb0;
-    const double* tape = D.ttape + (size_t)(c * 2 + set) * 9 * TT_LEN;
+    const double* tape = (set ? D.ttape_n : D.ttape) + (size_t)c * 9 * TT_LEN;
     double g_pr = 0.0;
 #pragma unroll 1
     for (int n = 0; n < 9; ++n) {

@@ -62,6 +62,16 @@ struct Instance {
     bool cn_alloc = false;        // S2CN2 target buffers allocated
     bool cn_targets_fresh = false; // tg holds the volume targets of the current state
     bool adj_alloc = false;       // S2CN2 tapes and reverse-step buffers allocated
+    // reverse segments: per-step forward buffers and descriptors
+    struct StepSet {
+        double *Wn, *hn, *bn, *tgn[6], *rh, *rb, *lines, *xf, *yf, *wb, *wt;
+        double *ftape, *wtape, *ttape_n, *ctape, *lim, *scal;
+    };
+    std::vector<StepSet> sets;
+    Dev* dsets = nullptr;         // device descriptors, one per step of a segment
+    double* dgwall = nullptr;     // [step][2][nx][4] adjoints of the wall fluxes
+    double* dsums = nullptr;      // [step][4] parameter gradients
+    int seg_cap = 0;
     int lo_wg = 256;              // workgroup size of the low-order Explicit kernels
 };
 
@@ -505,7 +515,8 @@ static int adj_buffers(Instance* I) {
     const size_t nfaces = nc + nx * (ny - 1);
     int rc;
     struct { double** p; size_t n; } bufs[] = {
-        {&D.ftape, nfaces * 3 * FT_LEN}, {&D.wtape, 2 * nx * WT_LEN}, {&D.ttape, nc * 2 * 9 * TT_LEN},
+        {&D.ftape, nfaces * 3 * FT_LEN}, {&D.wtape, 2 * nx * WT_LEN}, {&D.ttape, nc * 9 * TT_LEN},
+        {&D.ttape_n, nc * 9 * TT_LEN},
         {&D.ctape, nc * CT_LEN}, {&D.a_lamW, 4 * nc}, {&D.a_lamh, cv}, {&D.a_lamb, cv}, {&D.a_hs, cv},
         {&D.a_bs, cv}, {&D.a_xf, nc * REC_ROWS * nv}, {&D.a_yf, nx * (ny - 1) * REC_ROWS * nv},
         {&D.a_wb, nx * REC_ROWS * nv}, {&D.a_wt, nx * REC_ROWS * nv}, {&D.a_ws, nc * 96}, {&D.a_wv, nc * 36},
@@ -558,6 +569,114 @@ extern "C" int ugks_cn_adjoint_device(void* handle, const double* W, const doubl
     // the forward re-run must not advance the S2CN2 conservation ledger
     HIPCHK(hipMemcpyAsync(D.scal, scal, sizeof scal, hipMemcpyHostToDevice, I->stream));
     HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
+// per-step forward buffers for a reverse segment of n steps (kept between calls)
+static int seg_buffers(Instance* I, int n) {
+    Dev& D = I->D;
+    const size_t nc = D.ncell, nv = D.nvel, cv = nc * nv, nx = D.nx, ny = D.ny;
+    const size_t nfaces = nc + nx * (ny - 1);
+    int rc;
+    while ((int)I->sets.size() < n) {
+        Instance::StepSet S;
+        struct { double** p; size_t n; } bufs[] = {
+            {&S.Wn, 4 * nc}, {&S.hn, cv}, {&S.bn, cv}, {&S.rh, cv}, {&S.rb, cv}, {&S.lines, nc * LINE_ROWS * 2 * nv},
+            {&S.xf, nc * REC_ROWS * nv}, {&S.yf, nx * (ny - 1) * REC_ROWS * nv}, {&S.wb, nx * REC_ROWS * nv},
+            {&S.wt, nx * REC_ROWS * nv}, {&S.ftape, nfaces * 3 * FT_LEN}, {&S.wtape, 2 * nx * WT_LEN},
+            {&S.ttape_n, nc * 9 * TT_LEN}, {&S.ctape, nc * CT_LEN}, {&S.lim, cv}, {&S.scal, SC_COUNT}};
+        for (auto& q : bufs)
+            if ((rc = dmalloc(I, q.p, q.n))) return rc;
+        for (int q = 0; q < 6; ++q)
+            if ((rc = dmalloc(I, &S.tgn[q], cv))) return rc;
+        I->sets.push_back(S);
+    }
+    if (I->seg_cap < n) {
+        if ((rc = dmalloc(I, &I->dsets, (size_t)n)) || (rc = dmalloc(I, &I->dgwall, (size_t)n * 8 * nx)) ||
+            (rc = dmalloc(I, &I->dsums, (size_t)n * 4)))
+            return rc;
+        I->seg_cap = n;
+    }
+    return E_OK;
+}
+
+// descriptor of step k of an n-step segment: the step's own forward buffers,
+// the previous step's proposal as its state, and the adjoint buffers with the
+// roles of input and output alternating from step to step
+static Dev seg_desc(const Instance* I, int k, int n) {
+    Dev d = I->D;
+    const Instance::StepSet& S = I->sets[k];
+    if (k > 0) {
+        const Instance::StepSet& P = I->sets[k - 1];
+        d.W = P.Wn; d.h = P.hn; d.b = P.bn;
+        for (int q = 0; q < 6; ++q) d.tg[q] = P.tgn[q];
+        d.ttape = P.ttape_n;
+    }
+    d.Wn = S.Wn; d.hn = S.hn; d.bn = S.bn;
+    for (int q = 0; q < 6; ++q) d.tgn[q] = S.tgn[q];
+    d.rh = S.rh; d.rb = S.rb; d.lines = S.lines; d.xf = S.xf; d.yf = S.yf; d.wb = S.wb; d.wt = S.wt;
+    d.ftape = S.ftape; d.wtape = S.wtape; d.ttape_n = S.ttape_n; d.ctape = S.ctape; d.lim = S.lim; d.scal = S.scal;
+    d.a_gwall = I->dgwall + (size_t)k * 8 * d.nx;
+    d.a_sum = I->dsums + 4 * (size_t)k;
+    if ((n - 1 - k) & 1) {
+        std::swap(d.a_lamW, d.a_Wout);
+        std::swap(d.a_lamh, d.a_h);
+        std::swap(d.a_lamb, d.a_b);
+    }
+    return d;
+}
+
+// Reverse of n consecutive S2CN2 steps from the state S_0 (no counterpart in
+// the reference): the n forward steps are run once with their buffers kept,
+// then reversed in the opposite order.  lam*: adjoint of S_n; gwall: host
+// array [n][2][nx][4], the adjoints of each step's wall fluxes.  Out: the
+// adjoint of S_0 (device) and the summed parameter gradients gpar4 (host).
+// Device memory grows by about 70 doubles per cell and velocity per step.
+extern "C" int ugks_cn_adjoint_segment(void* handle, const double* W, const double* h, const double* b, int n,
+                                       const double* lamW, const double* lamh, const double* lamb,
+                                       const double* gwall, double* gW, double* gh, double* gb, double* gpar4) {
+    Instance* I = (Instance*)handle;
+    if (!I || !W || !h || !b || !lamW || !lamh || !lamb || !gwall || !gW || !gh || !gb || !gpar4)
+        return fail(E_ARG, "null argument");
+    if (n < 1) return fail(E_ARG, "a segment needs at least one step");
+    int rc;
+    if ((rc = cn_buffers(I)) || (rc = adj_buffers(I)) || (rc = seg_buffers(I, n))) return rc;
+    Dev& D = I->D;
+    const size_t nc = D.ncell, cv = nc * D.nvel;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(D.W, W, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.h, h, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.b, b, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamW, lamW, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamh, lamh, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(D.a_lamb, lamb, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(I->dgwall, gwall, sizeof(double) * 8 * D.nx * n, hipMemcpyHostToDevice, I->stream));
+    std::vector<Dev> hd(n);
+    for (int k = 0; k < n; ++k) hd[k] = seg_desc(I, k, n);
+    HIPCHK(hipMemcpyAsync(I->dsets, hd.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, I->stream));
+    I->operator_fresh = false;
+    I->cn_targets_fresh = false;
+    for (int k = 0; k < n; ++k) {
+        LaunchSet L = {&hd[k], I->dsets + k, 1};
+        HIPCHK(ugks_launch_cn(&L, k == 0 ? 1 : 0, I->stream, &I->prof));
+    }
+    if ((rc = check_status(I, nullptr))) return rc;
+    for (int k = n - 1; k >= 0; --k) {
+        LaunchSet L = {&hd[k], I->dsets + k, 1};
+        HIPCHK(ugks_launch_cn_adjoint(&L, I->stream, &I->prof));
+    }
+    const Dev& d0 = hd[0];
+    HIPCHK(hipMemcpyAsync(gW, d0.a_Wout, sizeof(double) * 4 * nc, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(gh, d0.a_h, sizeof(double) * cv, dd, I->stream));
+    HIPCHK(hipMemcpyAsync(gb, d0.a_b, sizeof(double) * cv, dd, I->stream));
+    std::vector<double> sums(4 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(sums.data(), I->dsums, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    for (int m = 0; m < 4; ++m) {
+        double t = 0.0;
+        for (int k = 0; k < n; ++k) t += sums[4 * k + m];
+        gpar4[m] = t;
+    }
     return E_OK;
 }
 

@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(NT) k_cn_targets(const Dev* __restrict__ DS) {
     double* amb = dyn + nvel;
     double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
     unsigned st = volume_targets<NT>(D, W, D.wv + (size_t)c * 36, node, amh, amb, lds,
-                                     D.ttape ? D.ttape + (size_t)(c * 2) * 9 * TT_LEN : nullptr);
+                                     D.ttape ? D.ttape + (size_t)c * 9 * TT_LEN : nullptr);
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;
@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(NT) k_cn_cell(const Dev* __restrict__ DS) {
     const double W[4] = {D.W[4 * c], D.W[4 * c + 1], D.W[4 * c + 2], D.W[4 * c + 3]};
     const double Wn[4] = {D.Wn[4 * c], D.Wn[4 * c + 1], D.Wn[4 * c + 2], D.Wn[4 * c + 3]};
     unsigned st = volume_targets<NT>(D, Wn, D.wv + (size_t)c * 36, node, amh, amb, lds,
-                                     D.ttape ? D.ttape + (size_t)(c * 2 + 1) * 9 * TT_LEN : nullptr);
+                                     D.ttape_n ? D.ttape_n + (size_t)c * 9 * TT_LEN : nullptr);
     if (st) {
         if (threadIdx.x == 0) atomicOr(D.status, st | ST_AT_CELL);
         return;

@@ -32,8 +32,8 @@ enum ScalarSlot {
 //              Shakhov moments mz_x(4) mz_y(4), q0(2), time-slope moments(4)
 //   wall face  [wall face][WT_LEN]: theta(4) J(16), then per node swt(4)
 //              hl(2) incidence reflection
-//   volume node [cell][set][node][TT_LEN]: theta(4) J(16) wq(4) tdx(4) tdy(4)
-//              (set 0: targets of the state, 1: of the proposal)
+//   volume node [cell][node][TT_LEN]: theta(4) J(16) wq(4) tdx(4) tdy(4), for
+//              the targets of the state (ttape) and of the proposal (ttape_n)
 //   CN cell    [cell][CT_LEN]: kappa(4) qm(2) rold(2) rq(2) rnew(2) beta_old beta
 enum { FT_LEN = 40, WT_LEN = 44, TT_LEN = 32, CT_LEN = 16, FJ_W = 12, WJ_W = 24 };
 
@@ -86,7 +86,7 @@ struct Dev {
     double *tg[6], *tgn[6];
     double *cnpart;       // [ncell][8] per-cell CN diagnostics
     // S2CN2 forward tapes (null unless the reverse step is in use)
-    double *ftape, *wtape, *ttape, *ctape;
+    double *ftape, *wtape, *ttape, *ttape_n, *ctape;
     // reverse step: adjoints of the proposal (inputs), of the streamed h, b,
     // of both target sets, of the face records, of the macro reconstruction
     // outputs, 5x5 block gradients, partial adjoints of W and Wn, parameter
