@@ -1,7 +1,7 @@
 """Throughput of the UGKS2D pseudo-time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|shipped]
-                    [--legs hoi,explicit,batched] [--no-cpu] [--copy-calibration]
+                    [--legs hoi,explicit,batched,adjoint] [--no-cpu] [--copy-calibration]
 
 Headline (`value`): one HighOrderImplicit pseudo-time iteration of the
 Couette solver at 64x512 cells x 64x64 velocities (physical operator: time
@@ -24,7 +24,12 @@ Two more legs go into the same JSON line:
 - "batched": 64 seeded Couette instances (nx = 8, ny in {32..128}, 32x32
   velocities, ragged Kn, wall Mach and wall temperature) advanced by one
   captured hipGraph per iteration after the autotuner picked the launch
-  shapes, against the plain per-instance loop on the same GPU.
+  shapes, against the plain per-instance loop on the same GPU;
+- "adjoint": gradients of the integrated wall shear plus heat flux of 100
+  S2CN2 steps on 16x128 cells x 32x32 velocities (Ma 2, Kn 0.1, seeded
+  perturbed equilibrium) with respect to the wall speed, mu_ref, Pr and f0,
+  with checkpoints every 10 steps: the HIP forward, the HIP reverse sweep,
+  their ratio, and one step of the torch autograd reverse for comparison.
 """
 from __future__ import annotations
 
@@ -75,7 +80,7 @@ def _args():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["large", "shipped"], default="large")
-    ap.add_argument("--legs", default="hoi,explicit,batched")
+    ap.add_argument("--legs", default="hoi,explicit,batched,adjoint")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -253,6 +258,53 @@ def leg_batched(U, iterations):
                         "temperature ratio uniform [0.8,1.2]; one captured hipGraph per iteration"}
 
 
+def leg_adjoint(nsteps=100, every=10):
+    import torch
+    from ugks2d_high_order_amd import adjoint as A
+    from ugks2d_high_order_amd import config, velocity
+    from ugks2d_high_order_amd.s2cn2_torch import S2CN2Model
+    case = config.adjoint_case()
+    M = S2CN2Model(case, *velocity.table(case.table, case.table_scale), device="cuda")
+    h0, b0 = A.perturbed_equilibrium(M)
+    f = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda", requires_grad=True)  # noqa: E731
+
+    def sync():
+        torch.cuda.synchronize()
+
+    def grads(K, backend="hip"):
+        hh, bb = h0.clone().requires_grad_(True), b0.clone().requires_grad_(True)
+        p = [f(case.wall_ux[1]), f(case.mu_ref), f(case.pr)]
+        sync()
+        t0 = time.perf_counter()
+        J = A.trajectory_objective(M, hh, bb, *p, K, every, backend=backend)
+        sync()
+        t1 = time.perf_counter()
+        J.backward()
+        sync()
+        t2 = time.perf_counter()
+        return J.item(), [x.grad.item() for x in p], t1 - t0, t2 - t1
+
+    grads(every)   # warm: instance, segment buffers, kernels
+    with torch.no_grad():
+        A.trajectory_objective(M, h0, b0, case.wall_ux[1], case.mu_ref, case.pr, 2, 2, backend="hip")
+        sync()
+        t0 = time.perf_counter()
+        A.trajectory_objective(M, h0, b0, case.wall_ux[1], case.mu_ref, case.pr, nsteps, every, backend="hip")
+        sync()
+        t_fwd = time.perf_counter() - t0
+    J, g, t_ckpt, t_bwd = grads(nsteps)
+    _, _, _, t_torch = grads(1, backend="torch")
+    cv = case.nx * case.ny * case.nu * case.nv
+    return {"workload": "16x128 cells x 32x32 velocities, Ma 2 walls at -U/+U, Kn 0.1, omega 0.81, Pr 2/3; "
+                        "f0 = discrete equilibrium x (1 + 1e-3 N(0,1)) (PCG64 seed 20261203); "
+                        f"{nsteps} S2CN2 steps, checkpoints every {every}",
+            "objective": J, "grad_wall_speed": g[0], "grad_mu_ref": g[1], "grad_pr": g[2],
+            "forward_s": t_fwd, "forward_with_checkpoints_s": t_ckpt, "backward_s": t_bwd,
+            "backward_over_forward": t_bwd / t_fwd, "cv_steps_per_s_forward": cv * nsteps / t_fwd,
+            "torch_autograd_reverse_s_per_step": t_torch,
+            "backward_speedup_vs_torch_autograd": t_torch * nsteps / t_bwd}
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -309,6 +361,8 @@ def main():
             "cpu_baseline": None if a.no_cpu else cpu_baseline_explicit(case, threads)}
     if "batched" in legs:
         out["batched"] = leg_batched(U, max(3, a.steps // 4))
+    if "adjoint" in legs:
+        out["adjoint"] = leg_adjoint()
     print(json.dumps(out), flush=True)
 
 

@@ -1,17 +1,18 @@
-"""The checkpointed adjoint over HIP forward steps (ugks_cn_step).
+"""The checkpointed adjoint with the HIP forward (ugks_cn_step) and the
+hand-written HIP reverse step (ugks_cn_adjoint_device,
+ugks_cn_adjoint_segment), against reverse-mode autograd of the FP64 torch
+restatement (itself pinned to the oracle's step) and central differences.
 
-- The HIP forward trajectory (4 S2CN2 steps) and the torch restatement's
-  trajectory on the GPU give the same objective (1e-12 relative) and the same
-  checkpointed states (1e-12 normwise relative).
-- Gradients with the HIP forward equal those with the torch forward (1e-10
-  relative: the recomputed states differ by round-off only).
-- Central differences of the HIP forward's objective (h = 1e-6 relative to
-  each parameter, and along a seeded relative perturbation of f0) agree with
-  the adjoint gradients to 1e-6 relative.
-
-Case: 8x16 cells x 16^2 velocities, Ma = 2 walls, Kn = 0.1, omega = 0.81,
-initial f = discrete equilibrium x (1 + 1e-3 N(0,1)), seeded; K = 4,
-checkpoints every 2 steps.
+Small case (8x16 cells x 16^2 velocities, Ma 2 walls, Kn 0.1, omega 0.81,
+f0 = discrete equilibrium x (1 + 1e-3 N(0,1)), seeded; K = 4, checkpoints
+every 2 steps):
+- the HIP and torch trajectories give the same objective (1e-12) and the same
+  checkpointed states (1e-12 normwise relative);
+- the HIP reverse sweep equals torch autograd (1e-10 relative);
+- central differences (h = 1e-6) agree with the gradients with respect to
+  the wall speed, mu_ref, Pr and a seeded f0 direction to 1e-6;
+- a reverse segment equals the single reverse steps bitwise.
+The adjoint workload (16x128 cells x 32^2 velocities) is checked below.
 """
 import math
 
@@ -64,7 +65,7 @@ def test_hip_and_torch_trajectories_agree(setup):
             assert e < 1e-12, e
 
 
-def test_hip_adjoint_matches_torch_adjoint_and_finite_differences(setup):
+def test_hip_reverse_matches_torch_autograd_and_finite_differences(setup):
     case, M, U0, h0, b0 = setup
     Jh, gh, hgh, bgh = _grads(M, case, U0, h0, b0, "hip")
     Jt, gt, hgt, bgt = _grads(M, case, U0, h0, b0, "torch")
@@ -95,3 +96,118 @@ def test_hip_adjoint_matches_torch_adjoint_and_finite_differences(setup):
     g = ((hgh * dh).sum() + (bgh * db).sum()).item()
     print("f0 direction FD", fd, "adjoint", g, flush=True)
     assert abs(fd - g) <= 1e-6 * abs(g)
+
+
+# ---------------------------------------------------------------------------
+# the adjoint workload: 16x128 cells x 32^2 velocities, Ma 2, Kn 0.1
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def workload():
+    case = config.adjoint_case()
+    M = S2CN2Model(case, *velocity.table(case.table, case.table_scale), device="cuda")
+    h0, b0 = A.perturbed_equilibrium(M)
+    return case, M, h0, b0
+
+
+def _wl_grads(case, M, h0, b0, K, every, backend="hip"):
+    f = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda", requires_grad=True)  # noqa: E731
+    hh, bb = h0.clone().requires_grad_(True), b0.clone().requires_grad_(True)
+    p = [f(case.wall_ux[1]), f(case.mu_ref), f(case.pr)]
+    J = A.trajectory_objective(M, hh, bb, *p, K, every, backend=backend)
+    J.backward()
+    return J.item(), [x.grad.item() for x in p], hh.grad, bb.grad
+
+
+def _bulk_direction(M, h0, b0, seed=7):
+    """Seeded relative perturbation of f0 on the interior rows (4..ny-5) and
+    the velocities with |u|, |v| < 6."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows = torch.zeros(M.ny, 1, 1, dtype=torch.float64, device="cuda")
+    rows[4:M.ny - 4] = 1.0
+    mask = rows * ((M.u.abs() < 6.0) & (M.v.abs() < 6.0)).double()[None, None, :]
+    dh = torch.tensor(rng.standard_normal(tuple(h0.shape)), device="cuda") * h0 * mask
+    db = torch.tensor(rng.standard_normal(tuple(b0.shape)), device="cuda") * b0 * mask
+    return dh, db
+
+
+def test_reverse_segment_equals_single_reverse_steps(setup):
+    """ugks_cn_adjoint_segment over 3 steps == 3 ugks_cn_adjoint_device calls, bitwise."""
+    case, M, U0, h0, b0 = setup
+    st = A.HipStepper(M)
+    params = (M.f64(U0), M.f64(case.mu_ref), M.f64(case.pr))
+    kept, _, _ = st.run(A.initial_state(M, h0, b0), h0, b0, params, 3, keep_every=1)
+    torch.manual_seed(3)
+    lam = [torch.randn_like(x) * 1e-2 for x in kept[3]]
+    gws = [torch.randn(2, case.nx, 4, dtype=torch.float64) for _ in range(3)]
+    l1, gps = lam, {}
+    for k in (2, 1, 0):
+        l1, gps[k] = st.reverse(kept[k], l1, gws[k], params)
+    gsum = np.zeros(4)
+    for k in (0, 1, 2):   # the library sums the steps' gradients in step order
+        gsum = gsum + gps[k]
+    l2, gp2 = st.reverse_segment(kept[0], 3, lam, gws, params)
+    for a, b in zip(l2, l1):
+        assert torch.equal(a, b)
+    assert np.array_equal(gp2, gsum)
+
+
+def test_workload_hip_adjoint_equals_torch_autograd(workload):
+    """10 steps of the adjoint workload, checkpoints every 5: the HIP reverse
+    sweep against reverse-mode autograd of the torch restatement.  Parameter
+    gradients 1e-12 relative; f0 gradients along two seeded directions (the
+    whole field, and the bulk) 1e-9 relative (measured 1e-12..2e-11)."""
+    case, M, h0, b0 = workload
+    Jh, gh, hgh, bgh = _wl_grads(case, M, h0, b0, 10, 5, "hip")
+    Jt, gt, hgt, bgt = _wl_grads(case, M, h0, b0, 10, 5, "torch")
+    assert abs(Jh - Jt) <= 1e-12 * abs(Jt)
+    for a, b in zip(gh, gt):
+        assert abs(a - b) <= 1e-12 * abs(b), (gh, gt)
+    rng = np.random.Generator(np.random.PCG64(11))
+    dirs = [(torch.tensor(rng.standard_normal(tuple(h0.shape)), device="cuda") * h0,
+             torch.tensor(rng.standard_normal(tuple(b0.shape)), device="cuda") * b0), _bulk_direction(M, h0, b0)]
+    for dh, db in dirs:
+        a = ((hgh * dh).sum() + (bgh * db).sum()).item()
+        b = ((hgt * dh).sum() + (bgt * db).sum()).item()
+        print("f0 direction hip", a, "torch", b, flush=True)
+        assert abs(a - b) <= 1e-9 * abs(b)
+
+
+def test_workload_gradients_against_finite_differences(workload):
+    """100 steps of the adjoint workload, checkpoints every 10.
+
+    Wall speed, mu_ref, Pr: fourth-order central differences of the HIP
+    forward with h = 3e-5 relative agree to 1e-6 (measured 3e-10, 3e-8, 7e-7).
+    f0 along the bulk direction: central differences with h = 1e-7 agree to
+    1e-4 (measured 3.7e-5).  The step is not differentiable everywhere (the
+    positivity limiter's min and the |beta_3 - beta_k| of the WENO5-AO
+    weights switch branches), so along f0 directions the difference
+    quotient's error grows linearly with h and with the number of steps
+    (10 steps: 7e-6 at h = 1e-7, 2.5e-5 at 1e-6, 5e-4 at 1e-5); the adjoint,
+    which equals torch autograd of the same map to 1e-11
+    (test_workload_hip_adjoint_equals_torch_autograd), is the derivative at
+    the point."""
+    case, M, h0, b0 = workload
+    K, every = 100, 10
+    J, g, hg, bg = _wl_grads(case, M, h0, b0, K, every)
+
+    def Jof(U, mu, pr, h_, b_):
+        with torch.no_grad():
+            return A.trajectory_objective(M, h_, b_, U, mu, pr, K, every).item()
+
+    base = [case.wall_ux[1], case.mu_ref, case.pr]
+    for i in range(3):
+        e = 3e-5 * abs(base[i])
+        vals = []
+        for s in (2, 1, -1, -2):
+            bp = list(base)
+            bp[i] += s * e
+            vals.append(Jof(*bp, h0, b0))
+        fd = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * e)
+        print("parameter", i, "FD4", fd, "adjoint", g[i], flush=True)
+        assert abs(fd - g[i]) <= 1e-6 * abs(g[i])
+    dh, db = _bulk_direction(M, h0, b0)
+    gd = ((hg * dh).sum() + (bg * db).sum()).item()
+    e = 1e-7
+    fd = (Jof(*base, h0 + e * dh, b0 + e * db) - Jof(*base, h0 - e * dh, b0 - e * db)) / (2 * e)
+    print("f0 bulk direction FD", fd, "adjoint", gd, flush=True)
+    assert abs(fd - gd) <= 1e-4 * abs(gd)

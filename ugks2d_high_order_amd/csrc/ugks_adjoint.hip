@@ -1125,55 +1125,82 @@ __global__ void __launch_bounds__(128) k_adj_limiter(const Dev* __restrict__ DS)
     const double keep = 1.0 - 64.0 * UGKS_DBL_EPS, den = m - f;
     const double gm = gth * keep * (-f) / (den * den);
     const double gf = gth * keep * m / (den * den);
-    double gside[4][3][3], gvol[3][3];
+    // reverse of the single candidate value through both stages
+    int cols[5];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-#pragma unroll
-            for (int e = 0; e < 3; ++e) gside[s][a][e] = 0.0;
-            if (arg == s * 3 + a) gside[s][a][0] = gf;
-        }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) gvol[a][b] = arg == 12 + a * 3 + b ? gf : 0.0;
-    FirstLayer L, gL;
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        const double* p = D.lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
-        L.xv[r][0] = p[0]; L.xv[r][1] = p[row]; L.xd[r][0] = p[2 * row]; L.xd[r][1] = p[3 * row];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) L.rows[r][a] = p[(4 + a) * row];
+    for (int q = 0; q < 5; ++q) {
+        int col = i + q - 2;
+        cols[q] = col < 0 ? col + nx : (col >= nx ? col - nx : col);
     }
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        int col = i + s - 2;
-        col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
-        const double* p = D.lines + (size_t)(j * nx + col) * LINE_ROWS * row + t;
-        L.yv[s][0] = p[7 * row]; L.yv[s][1] = p[8 * row]; L.yd[s][0] = p[9 * row]; L.yd[s][1] = p[10 * row];
-    }
-    zero_layer(gL);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) side_adj(KA, L, near, sh, s, gside[s], gL);
-    vol_adj(KA, L, near, sh, gvol, gL);
-    double blk[5][5], gblk[5][5];
+    double gb[5][5];
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            int col = i + s - 2;
-            col = col < 0 ? col + nx : (col >= nx ? col - nx : col);
-            blk[r][s] = F[(size_t)((rb + r) * nx + col) * nv + k];
-            gblk[r][s] = 0.0;
+        for (int q = 0; q < 5; ++q) gb[r][q] = 0.0;
+    if (arg < 6 || arg >= 12) {
+        // an x-side value (side s, node a) or a volume node (x node a, y node b): both are
+        // reconstructions in y of the column stencil's x lines at a point px of each line
+        int px, py;
+        if (arg < 6) { px = arg / 3; py = 2 + arg % 3; }
+        else { px = 2 + (arg - 12) / 3; py = 2 + (arg - 12) % 3; }
+        double q[5], gq[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const double* p = D.lines + (size_t)((rb + r) * nx + i) * LINE_ROWS * row + t;
+            q[r] = px < 2 ? p[px * row] : p[(2 + px) * row];   // face values rows 0,1; Gauss values rows 4..6
         }
-    first_layer_adj(KA, blk, near, sh, gL, gblk);
-    gblk[j - rb][2] += gm;
+        if (near) {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) gq[r] = gf * KA.wv[sh][py][r];
+        } else {
+            const int pts[1] = {py};
+            const double gv[1] = {gf}, gd[1] = {0.0};
+            weno_pts_adj<1>(KA, q, pts, gv, gd, gq);
+        }
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            double q5[5];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) q5[s] = F[(size_t)((rb + r) * nx + cols[s]) * nv + k];
+            const int pts[1] = {px};
+            const double gv[1] = {gq[r]}, gd[1] = {0.0};
+            weno_pts_adj<1>(KA, q5, pts, gv, gd, gb[r]);
+        }
+    } else {
+        // a y-side value (side 2 + py0, node a): a reconstruction in x of the row
+        // stencil's y lines at their face py0
+        const int py0 = (arg - 6) / 3, a = (arg - 6) % 3;
+        double q[5], gq[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 5; ++s) q[s] = D.lines[(size_t)(j * nx + cols[s]) * LINE_ROWS * row + (7 + py0) * row + t];
+        const int pts[1] = {2 + a};
+        const double gv[1] = {gf}, gd[1] = {0.0};
+        weno_pts_adj<1>(KA, q, pts, gv, gd, gq);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            double q5[5], g5[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 5; ++r) q5[r] = F[(size_t)((rb + r) * nx + cols[s]) * nv + k];
+            if (near) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) g5[r] = gq[s] * KA.wv[sh][py0][r];
+            } else {
+                const int p1[1] = {py0};
+                const double v1[1] = {gq[s]}, d1[1] = {0.0};
+                weno_pts_adj<1>(KA, q5, p1, v1, d1, g5);
+            }
+#pragma unroll
+            for (int r = 0; r < 5; ++r) gb[r][s] += g5[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+        if (r == j - rb) gb[r][2] += gm;
     double* o = D.a_blk + (size_t)c * 25 * row + t;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-        for (int s = 0; s < 5; ++s) o[(size_t)(r * 5 + s) * row] += gblk[r][s];
+        for (int s = 0; s < 5; ++s) o[(size_t)(r * 5 + s) * row] += gb[r][s];
 }
 
 // lam_h, lam_b: direct part + streamed part + gathered block adjoints
