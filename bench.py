@@ -29,7 +29,12 @@ Two more legs go into the same JSON line:
   S2CN2 steps on 16x128 cells x 32x32 velocities (Ma 2, Kn 0.1, seeded
   perturbed equilibrium) with respect to the wall speed, mu_ref, Pr and f0,
   with checkpoints every 10 steps: the HIP forward, the HIP reverse sweep,
-  their ratio, and one step of the torch autograd reverse for comparison.
+  their ratio, and one step of the torch autograd reverse for comparison;
+- "calibration" (only with --legs calibration, about 10 minutes): 256 seeded
+  Couette instances at 8x64 cells x 32x32 velocities recover hidden
+  (Kn, wall Mach) from their velocity and temperature profiles after 20
+  S2CN2 steps by 50 quasi-Newton iterations on the batched forward and the
+  batched adjoint; recovery errors and the time per iteration.
 """
 from __future__ import annotations
 
@@ -305,6 +310,21 @@ def leg_adjoint(nsteps=100, every=10):
             "backward_speedup_vs_torch_autograd": t_torch * nsteps / t_bwd}
 
 
+def leg_calibration(iterations=50):
+    from ugks2d_high_order_amd import calibrate
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    res = calibrate.calibrate(n=256, steps=20, every=5, iterations=iterations, chunk=32, log=log)
+    hist = res.pop("objective_median_history")
+    res["objective_median_first_last"] = [hist[0], hist[-1]]
+    res["workload"] = ("256 Couette instances, 8x64 cells x 32x32 velocities, CFL 0.4, rest start at T=2; hidden "
+                       "Kn log-uniform [0.001, 0.01] and wall Mach uniform [0.5, 2] (PCG64 seed 20261203); "
+                       "objective: x-averaged u(y), T(y) after 20 S2CN2 steps; start Kn 0.0032, Ma 1.25")
+    return res
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -363,6 +383,8 @@ def main():
         out["batched"] = leg_batched(U, max(3, a.steps // 4))
     if "adjoint" in legs:
         out["adjoint"] = leg_adjoint()
+    if "calibration" in legs:
+        out["calibration"] = leg_calibration()
     print(json.dumps(out), flush=True)
 
 

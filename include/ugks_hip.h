@@ -141,6 +141,11 @@ int ugks_cn_adjoint_segment(void* handle, const double* W, const double* h, cons
                             const double* lamW, const double* lamh, const double* lamb, const double* gwall,
                             double* gW, double* gh, double* gb, double* gpar4);
 
+/* Wall speeds (bottom, top) and reference viscosity mu_ref of an instance,
+ * from its next step on (no counterpart in the reference: the parameters of
+ * the calibration workload). */
+int ugks_set_physics(void* handle, double mu_ref, double wall_ux_bottom, double wall_ux_top);
+
 /* Low-order Explicit entry (TimeIntegration_method = 'Explicit'): Van Leer
  * slopes, the analytic-moment UGKS face flux with the projected face
  * equilibrium and its Shakhov part, the multiscale diffuse walls and the
@@ -231,6 +236,26 @@ int ugks_batch_set_shape(void* batch, int wg_vel, int tile);
  * (wg_vel, tile); ms9 (may be NULL): ms per iteration of each candidate,
  * row-major over wg_vel {64,128,256} x tile {8,16,32}. */
 int ugks_batch_autotune(void* batch, int reps, int* out2, double* ms9);
+
+/* S2CN2 over a batch of instances of one size (no counterpart in the
+ * reference; the batched forward and adjoint of the calibration workload).
+ * ugks_batch_set_physics: ugks_set_physics of member i (use it rather than
+ * ugks_set_physics on a member's handle, so the batch sees the change).
+ * States and adjoints are device arrays holding every instance's array back
+ * to back in the ugks_get_state layout.
+ * ugks_batch_cn_forward: n steps of every instance from (W, h, b) into
+ * (Wo, ho, bo), taken whatever the S2CN2 gates say.
+ * ugks_batch_cn_adjoint_segment: reverse of n steps of every instance from
+ * (W, h, b) given the adjoints of the states n steps later and of the wall
+ * fluxes (host [instance][n][2][nx][4], or NULL); out: the adjoints of
+ * (W, h, b) and on the host gpar[instance][4] = d/d(mu_ref, Pr, bottom wall
+ * speed, top wall speed). */
+int ugks_batch_set_physics(void* batch, int i, double mu_ref, double wall_ux_bottom, double wall_ux_top);
+int ugks_batch_cn_forward(void* batch, int n, const double* W, const double* h, const double* b, double* Wo,
+                          double* ho, double* bo);
+int ugks_batch_cn_adjoint_segment(void* batch, int n, const double* W, const double* h, const double* b,
+                                  const double* lamW, const double* lamh, const double* lamb, const double* gwall,
+                                  double* gW, double* gh, double* gb, double* gpar);
 
 #ifdef __cplusplus
 }

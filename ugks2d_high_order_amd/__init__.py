@@ -26,7 +26,8 @@ EXPORTS = [
     "ugks_explicit_step", "ugks_explicit_cold_start",
     "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
     "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
-    "ugks_cn_adjoint_device", "ugks_cn_adjoint_segment",
+    "ugks_cn_adjoint_device", "ugks_cn_adjoint_segment", "ugks_set_physics", "ugks_batch_cn_forward",
+    "ugks_batch_cn_adjoint_segment", "ugks_batch_set_physics",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -80,6 +81,10 @@ def load_library(path: str = LIB_PATH):
     L.ugks_explicit_step.argtypes = [C.c_void_p, _D, _D]
     L.ugks_wall_fluxes.argtypes = [C.c_void_p, _D]
     L.ugks_cn_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_D] + [C.c_void_p] * 3 + [_D]
+    L.ugks_set_physics.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.ugks_batch_cn_forward.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+    L.ugks_batch_set_physics.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ugks_batch_cn_adjoint_segment.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [_D] + [C.c_void_p] * 3 + [_D]
     L.ugks_cn_adjoint_segment.argtypes = ([C.c_void_p] + [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 + [_D]
                                           + [C.c_void_p] * 3 + [_D])
     L.ugks_explicit_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
@@ -270,6 +275,11 @@ class UGKS:
                                                        gh, gb, _ptr(out)))
         return out
 
+    def set_physics(self, mu_ref: float, wall_ux_bottom: float, wall_ux_top: float):
+        """Reference viscosity and wall speeds from the next step on."""
+        _check(load_library().ugks_set_physics(self.handle, float(mu_ref), float(wall_ux_bottom),
+                                               float(wall_ux_top)))
+
     def run(self, n: int) -> int:
         done = C.c_int(0)
         _check(load_library().ugks_run(self.handle, int(n), C.byref(done)))
@@ -326,9 +336,14 @@ class _InstanceView(UGKS):
         self.ncell = self.case.nx * self.case.ny
         self.handle = handle
         self._batch = batch
+        self._index = i
 
     def close(self):
         self.handle = None
+
+    def set_physics(self, mu_ref: float, wall_ux_bottom: float, wall_ux_top: float):
+        _check(load_library().ugks_batch_set_physics(self._batch.handle, self._index, float(mu_ref),
+                                                     float(wall_ux_bottom), float(wall_ux_top)))
 
 
 class Batch:
@@ -352,6 +367,26 @@ class Batch:
         _check(L.ugks_batch_create(n, self._p, _ptr(self.u), _ptr(self.v), _ptr(self.w), C.byref(h)))
         self.handle = h
         self.n = n
+
+    def cn_forward(self, n: int, W: int, h: int, b: int, Wo: int, ho: int, bo: int):
+        """n S2CN2 steps of every instance (same size) from the device arrays
+        (W, h, b) into (Wo, ho, bo), instance after instance back to back."""
+        _check(load_library().ugks_batch_cn_forward(self.handle, int(n), W, h, b, Wo, ho, bo))
+
+    def cn_adjoint_segment(self, n: int, W: int, h: int, b: int, lamW: int, lamh: int, lamb: int, gW: int, gh: int,
+                           gb: int, gwall=None):
+        """Reverse of n S2CN2 steps of every instance (device arrays back to
+        back); gwall: None or a host array [instances, n, 2, nx, 4].  Returns
+        [instances, 4] gradients with respect to (mu_ref, Pr, bottom and top
+        wall speed)."""
+        out = np.zeros((self.n, 4))
+        g = None
+        if gwall is not None:
+            g = np.ascontiguousarray(gwall, dtype=np.float64)
+        _check(load_library().ugks_batch_cn_adjoint_segment(self.handle, int(n), W, h, b, lamW, lamh, lamb,
+                                                             _ptr(g) if g is not None else None, gW, gh, gb,
+                                                             _ptr(out)))
+        return out
 
     def close(self):
         if getattr(self, "handle", None):

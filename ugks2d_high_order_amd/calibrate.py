@@ -1,0 +1,219 @@
+"""Batched adjoint calibration of Couette instances.
+
+Each instance is a plane Couette channel (walls at -U, +U, diffuse, at the
+initial gas temperature) started from the rest equilibrium and advanced K
+S2CN2 steps.  Synthetic target profiles, the x-averaged velocity u(y) and
+temperature T(y) after K steps, are made by the same forward at hidden
+(Kn, U) drawn from a seed.  Every instance then recovers its (Kn, U) by
+gradient descent on
+
+    J = sum_y ((u - u*) / u_ref)^2 + ((T - T*) / T_ref)^2,
+
+with the gradient from the batched forward (ugks_batch_cn_forward, states
+kept every `every` steps) and the batched hand-written adjoint
+(ugks_batch_cn_adjoint_segment): dJ/dmu_ref and dJ/dU come out of the reverse
+sweep, with the adjoint of the final state from J.  The descent runs in
+(log Kn, Ma): a BFGS quasi-Newton step per instance inside a trust radius,
+one gradient per iteration.
+
+Instances are processed in chunks that reuse one Batch of `chunk` instances
+(same size, own wall speed and Kn per chunk member), so device memory is that
+of `chunk` instances, whatever the number of cases.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import config, velocity
+
+
+def calibration_cases(n: int = 256, seed: int = 20261203, nx: int = 8, ny: int = 64, nu: int = 32,
+                      kn_range=(0.001, 0.01), ma_range=(0.5, 2.0), t_init: float = 2.0, cfl: float = 0.4):
+    """The base case (shape, gas, initial temperature, CFL) and the hidden
+    (Kn, Ma) of n instances: Kn log-uniform, wall Mach uniform, PCG64 seeded.
+    The profiles depend on Kn only where the run covers a collision time or
+    more (tau = mu_ref/p, mu_ref = Kn/sqrt(pi)): at 8x64 cells, CFL 0.4 and 20
+    steps the run lasts about 4e-3, i.e. 0.7 to 7 collision times over the
+    default Kn range."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kn = np.exp(rng.uniform(math.log(kn_range[0]), math.log(kn_range[1]), size=n))
+    ma = rng.uniform(ma_range[0], ma_range[1], size=n)
+    base = config.CouetteCase(nx=nx, ny=ny, nu=nu, nv=nu, table=f"half{nu}", omega=0.81, t_init=t_init,
+                              wall_tn=(t_init, t_init), mu_ref=0.1 / math.sqrt(math.pi), cfl=cfl,
+                              wall_ux=(-config.wall_speed(1.0, t_init), config.wall_speed(1.0, t_init)))
+    return base, kn, ma
+
+
+class Calibration:
+    """Forward, objective and adjoint of many instances of one shape."""
+
+    def __init__(self, base, n_cases: int, steps: int, every: int, chunk: int = 64, device: str = "cuda"):
+        from . import Batch
+        self.base, self.n, self.K, self.every = base, n_cases, steps, every
+        if steps % every:
+            raise ValueError("steps must be a multiple of every")
+        self.chunk = min(chunk, n_cases)
+        self.dev = torch.device(device)
+        self.u, self.v, self.w = velocity.table(base.table, base.table_scale)
+        self.B = Batch([base] * self.chunk, self.u, self.v, self.w)
+        self.views = [self.B.instance(i) for i in range(self.chunk)]
+        self.nc, self.nvel = base.nx * base.ny, self.u.size
+        self.gamma = (base.ck + 4) / (base.ck + 2)
+        # the rest equilibrium the runs start from (cold start of the first member)
+        self.B.cold_start()
+        W, h, b = self.views[0].state()
+        self.S0 = [torch.as_tensor(x, device=self.dev) for x in (W, h, b)]
+
+    def wall_speed(self, ma):
+        return ma * math.sqrt(self.gamma * 0.5 * self.base.t_init)
+
+    def _set_chunk(self, kn, ma):
+        for i, g in enumerate(self.views):
+            if i < len(kn):
+                U = self.wall_speed(float(ma[i]))
+                g.set_physics(float(kn[i]) / math.sqrt(math.pi), -U, U)
+
+    def _stack(self, m, S):
+        return [x.unsqueeze(0).expand(m, *x.shape).contiguous() for x in S]
+
+    def _pad(self, S, m):
+        """Arrays of a chunk of m members padded to the batch size."""
+        if m == self.chunk:
+            return [x.contiguous() for x in S]
+        return [torch.cat([x, x[:1].expand(self.chunk - m, *x.shape[1:])], 0).contiguous() for x in S]
+
+    def _forward(self, S):
+        out = [torch.empty_like(x) for x in S]
+        torch.cuda.synchronize(self.dev)
+        self.B.cn_forward(self.every, *[x.data_ptr() for x in S], *[x.data_ptr() for x in out])
+        return out
+
+    def profiles(self, W):
+        """x-averaged velocity u(y) and temperature T(y) (T_native) of W [m, nc, 4]."""
+        W = W.reshape(W.shape[0], self.base.ny, self.base.nx, 4)
+        rho = W[..., 0]
+        ux, vy = W[..., 1] / rho, W[..., 2] / rho
+        T = 4.0 * (W[..., 3] / rho - 0.5 * (ux * ux + vy * vy)) / (self.base.ck + 2)
+        return ux.mean(-1), T.mean(-1)
+
+    def run(self, kn, ma, target=None, grad=False):
+        """Final profiles of all cases at (kn, ma); with grad, also J per case
+        and dJ/d(log Kn, Ma)."""
+        prof_u, prof_T, Js, G = [], [], [], []
+        for c0 in range(0, self.n, self.chunk):
+            m = min(self.chunk, self.n - c0)
+            self._set_chunk(kn[c0:c0 + m], ma[c0:c0 + m])
+            S = self._pad(self._stack(m, self.S0), m)
+            kept = [S]
+            for _ in range(self.K // self.every):
+                S = self._forward(S)
+                kept.append(S)
+            W = S[0][:m]
+            pu, pT = self.profiles(W)
+            prof_u.append(pu)
+            prof_T.append(pT)
+            if not grad:
+                continue
+            Wg = W.detach().clone().requires_grad_(True)
+            u, T = self.profiles(Wg)
+            tu, tT = target[0][c0:c0 + m], target[1][c0:c0 + m]
+            J = (((u - tu) / self.u_ref) ** 2).sum(-1) + (((T - tT) / self.base.t_init) ** 2).sum(-1)
+            (gW,) = torch.autograd.grad(J.sum(), Wg)
+            lam = self._pad([gW, torch.zeros(m, self.nc, self.nvel, dtype=torch.float64, device=self.dev),
+                             torch.zeros(m, self.nc, self.nvel, dtype=torch.float64, device=self.dev)], m)
+            if m < self.chunk:
+                for x in lam:
+                    x[m:] = 0.0
+            gp = np.zeros((self.chunk, 4))
+            for seg in reversed(range(self.K // self.every)):
+                out = [torch.empty_like(x) for x in lam]
+                torch.cuda.synchronize(self.dev)
+                gp += self.B.cn_adjoint_segment(self.every, *[x.data_ptr() for x in kept[seg]],
+                                                *[x.data_ptr() for x in lam], *[x.data_ptr() for x in out])
+                lam = out
+            g_mu = gp[:m, 0]
+            g_U = -gp[:m, 2] + gp[:m, 3]
+            knc, mac = np.asarray(kn[c0:c0 + m]), np.asarray(ma[c0:c0 + m])
+            dlogkn = knc * g_mu / math.sqrt(math.pi)
+            dma = g_U * self.wall_speed(1.0)
+            Js.append(J.detach().cpu().numpy())
+            G.append(np.stack([dlogkn, dma], -1))
+        pu, pT = torch.cat(prof_u), torch.cat(prof_T)
+        if not grad:
+            return pu, pT
+        return pu, pT, np.concatenate(Js), np.concatenate(G)
+
+    @property
+    def u_ref(self):
+        return self.wall_speed(1.0)
+
+    def close(self):
+        self.B.close()
+
+
+def calibrate(n: int = 256, steps: int = 20, every: int = 5, iterations: int = 50, chunk: int = 64,
+              radius0: float = 0.3, seed: int = 20261203, kn0: float = 0.0032, ma0: float = 1.25, log=None,
+              **shape):
+    """Recover the hidden (Kn, Ma) of n seeded instances by gradient-based
+    descent in (log Kn, Ma): per instance a BFGS quasi-Newton step limited to
+    a trust radius, one forward + adjoint gradient per iteration; a step that
+    does not lower J is undone and the radius halved, an accepted one grows it
+    by 1.5.  Returns a dict with the recovery errors, the objective history and
+    the time per iteration."""
+    base, kn_true, ma_true = calibration_cases(n, seed, **shape)
+    cal = Calibration(base, n, steps, every, chunk)
+    target = cal.run(kn_true, ma_true)
+    theta = np.stack([np.full(n, math.log(kn0)), np.full(n, ma0)], -1)
+    best_J = np.full(n, np.inf)
+    best_theta, best_g = theta.copy(), np.zeros_like(theta)
+    H = np.zeros((n, 2, 2))
+    radius = np.full(n, radius0)
+    hist, times = [], []
+    cand = theta.copy()
+    for it in range(iterations):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, J, G = cal.run(np.exp(cand[:, 0]), cand[:, 1], target, grad=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        for i in range(n):
+            if J[i] < best_J[i]:
+                if np.isfinite(best_J[i]):
+                    s_ = cand[i] - best_theta[i]
+                    y_ = G[i] - best_g[i]
+                    sy = float(s_ @ y_)
+                    if sy > 1e-300:
+                        rho = 1.0 / sy
+                        I2 = np.eye(2)
+                        H[i] = (I2 - rho * np.outer(s_, y_)) @ H[i] @ (I2 - rho * np.outer(y_, s_)) + rho * np.outer(s_, s_)
+                    radius[i] *= 1.5
+                else:
+                    H[i] = np.diag(radius0 / np.maximum(np.abs(G[i]), 1e-300))
+                best_J[i], best_theta[i], best_g[i] = J[i], cand[i], G[i]
+            else:
+                radius[i] *= 0.5
+            d = -H[i] @ best_g[i]
+            nd = float(np.linalg.norm(d))
+            if not np.isfinite(nd) or nd == 0.0:
+                d, nd = -best_g[i] / max(float(np.linalg.norm(best_g[i])), 1e-300) * radius[i], radius[i]
+            if nd > radius[i]:
+                d *= radius[i] / nd
+            cand[i] = best_theta[i] + d
+            cand[i, 1] = min(max(cand[i, 1], 0.05), 4.0)
+        hist.append(float(np.median(best_J)))
+        if log:
+            log(f"iteration {it + 1}: median J {hist[-1]:.3e}, {times[-1]:.2f} s")
+    kn_fit, ma_fit = np.exp(best_theta[:, 0]), best_theta[:, 1]
+    err_kn = np.abs(kn_fit / kn_true - 1)
+    err_ma = np.abs(ma_fit / ma_true - 1)
+    cal.close()
+    return {"instances": n, "steps": steps, "every": every, "iterations": iterations, "chunk": chunk,
+            "shape": dataclasses.asdict(base), "objective_median_history": hist,
+            "s_per_iteration": float(np.median(times)), "kn_rel_error_median": float(np.median(err_kn)),
+            "kn_rel_error_max": float(err_kn.max()), "ma_rel_error_median": float(np.median(err_ma)),
+            "ma_rel_error_max": float(err_ma.max())}

@@ -1218,8 +1218,9 @@ __global__ void __launch_bounds__(256) k_adj_gather(const Dev* __restrict__ DS) 
 
 // parameter gradients: fixed-order sum of the per-cell, per-face and
 // per-wall partials (one workgroup)
-__global__ void __launch_bounds__(256) k_adj_sum(const Dev* __restrict__ DS, int nslots) {
+__global__ void __launch_bounds__(256) k_adj_sum(const Dev* __restrict__ DS, int) {
     const Dev& D = DS[blockIdx.z];
+    const int nslots = D.ncell + (D.ncell + D.nx * (D.ny - 1)) + 2 * D.nx;
     __shared__ double lds[16 * 4];
     double s[4] = {0, 0, 0, 0};
     for (int q = threadIdx.x; q < nslots; q += blockDim.x)

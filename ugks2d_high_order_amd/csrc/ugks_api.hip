@@ -680,6 +680,27 @@ extern "C" int ugks_cn_adjoint_segment(void* handle, const double* W, const doub
     return E_OK;
 }
 
+// Wall speeds and reference viscosity of an instance (no counterpart in the
+// reference; the calibration's parameters).  Takes effect from the next step.
+extern "C" int ugks_set_physics(void* handle, double mu_ref, double wall_ux_bottom, double wall_ux_top) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    if (!(mu_ref > 0.0) || !std::isfinite(wall_ux_bottom) || !std::isfinite(wall_ux_top))
+        return fail(E_ARG, "mu_ref must be positive and the wall speeds finite");
+    HIPCHK(hipStreamSynchronize(I->stream));
+    I->D.gas.mu_ref = mu_ref;
+    I->D.wall_ux[0] = wall_ux_bottom;
+    I->D.wall_ux[1] = wall_ux_top;
+    I->p.mu_ref = mu_ref;
+    I->p.wall_ux[0] = wall_ux_bottom;
+    I->p.wall_ux[1] = wall_ux_top;
+    I->operator_fresh = false;
+    int rc = upload(I);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
 // the low-order Explicit kernels keep Gaussian tables per axis (Nu, Nv <= 64,
 // powers of two)
 static int lo_check(const Instance* I) {
@@ -890,6 +911,9 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool device_truth = false;      // dDs holds the current descriptors
+    Dev* dcn = nullptr;             // S2CN2 step descriptors (one per instance)
+    Dev* dseg = nullptr;            // reverse-segment descriptors [step][instance]
+    int dseg_cap = 0;
 };
 
 void batch_drop_graph(Batch* B) {
@@ -1024,6 +1048,8 @@ extern "C" void ugks_batch_destroy(void* batch) {
     if (B->dDs) hipFree(B->dDs);
     if (B->dcount) hipFree(B->dcount);
     if (B->dstat) hipFree(B->dstat);
+    if (B->dcn) hipFree(B->dcn);
+    if (B->dseg) hipFree(B->dseg);
     if (B->stream) hipStreamDestroy(B->stream);
     delete B;
 }
@@ -1179,4 +1205,167 @@ extern "C" int ugks_batch_autotune(void* batch, int reps, int* out2, double* ms9
     out2[0] = bw;
     out2[1] = bt;
     return E_OK;
+}
+
+// ---------------------------------------------------------------------------
+// S2CN2 forward steps and reverse segments of a whole batch (no counterpart in
+// the reference).  The instances must share their size; states and adjoints
+// are device arrays of all instances back to back ([instance][...] in the
+// ugks_get_state layout of one instance).
+// ---------------------------------------------------------------------------
+static int batch_uniform(const Batch* B) {
+    const Dev& d0 = B->inst[0]->D;
+    for (const Instance* I : B->inst)
+        if (I->D.ncell != d0.ncell || I->D.ny != d0.ny) return fail(E_ARG, "batched S2CN2 needs instances of one size");
+    return E_OK;
+}
+
+// the statuses of every instance after work on B->stream
+static int batch_status(Batch* B) {
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (Instance* I : B->inst) {
+        int rc = check_status(I, nullptr);
+        if (rc) return rc;
+    }
+    return E_OK;
+}
+
+// n S2CN2 steps of every instance from the states (W, h, b) into (Wo, ho, bo).
+// The steps are taken whatever the S2CN2 gates say (a device failure still
+// fails the call).
+extern "C" int ugks_batch_cn_forward(void* batch, int n, const double* W, const double* h, const double* b,
+                                     double* Wo, double* ho, double* bo) {
+    Batch* B = (Batch*)batch;
+    if (!B || !W || !h || !b || !Wo || !ho || !bo || n < 0) return fail(E_ARG, "invalid argument");
+    int rc;
+    if ((rc = batch_uniform(B)) || (rc = batch_pull(B))) return rc;
+    const int ni = B->n;
+    const size_t nc = B->inst[0]->D.ncell, cv = nc * B->inst[0]->D.nvel;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    if (!B->dcn && hipMalloc(&B->dcn, sizeof(Dev) * ni) != hipSuccess) return fail(E_DEVICE, "allocation failed");
+    for (int i = 0; i < ni; ++i) {
+        Instance* I = B->inst[i];
+        if ((rc = cn_buffers(I))) return rc;
+        HIPCHK(hipStreamSynchronize(I->stream));
+        HIPCHK(hipMemcpyAsync(I->D.W, W + 4 * nc * i, sizeof(double) * 4 * nc, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(I->D.h, h + cv * i, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(I->D.b, b + cv * i, sizeof(double) * cv, dd, B->stream));
+        I->operator_fresh = false;
+    }
+    std::vector<Dev> hd(ni);
+    for (int k = 0; k < n; ++k) {
+        for (int i = 0; i < ni; ++i) hd[i] = B->inst[i]->D;
+        HIPCHK(hipMemcpyAsync(B->dcn, hd.data(), sizeof(Dev) * ni, hipMemcpyHostToDevice, B->stream));
+        LaunchSet L = {&hd[0], B->dcn, ni};
+        HIPCHK(ugks_launch_cn(&L, k == 0 ? 1 : 0, B->stream, nullptr));
+        if ((rc = batch_status(B))) return rc;
+        for (Instance* I : B->inst) {
+            Dev& D = I->D;
+            std::swap(D.W, D.Wn);
+            std::swap(D.h, D.hn);
+            std::swap(D.b, D.bn);
+            for (int q = 0; q < 6; ++q) std::swap(D.tg[q], D.tgn[q]);
+        }
+    }
+    for (int i = 0; i < ni; ++i) {
+        Instance* I = B->inst[i];
+        HIPCHK(hipMemcpyAsync(Wo + 4 * nc * i, I->D.W, sizeof(double) * 4 * nc, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(ho + cv * i, I->D.h, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(bo + cv * i, I->D.b, sizeof(double) * cv, dd, B->stream));
+        I->cn_targets_fresh = n > 0;
+        if ((rc = upload(I))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(B->stream));
+    B->device_truth = false;
+    return batch_push(B);
+}
+
+// Reverse of n S2CN2 steps of every instance from the states (W, h, b), given
+// the adjoints of the states n steps later (lamW, lamh, lamb) and of the wall
+// fluxes (host [instance][step][2][nx][4], or null for none).  Out: the
+// adjoints of the states and, on the host, gpar[instance][4] = d/d(mu_ref, Pr,
+// bottom wall speed, top wall speed) summed over the steps.  Device memory per
+// instance as for ugks_cn_adjoint_segment.
+extern "C" int ugks_batch_cn_adjoint_segment(void* batch, int n, const double* W, const double* h, const double* b,
+                                             const double* lamW, const double* lamh, const double* lamb,
+                                             const double* gwall, double* gW, double* gh, double* gb,
+                                             double* gpar) {
+    Batch* B = (Batch*)batch;
+    if (!B || !W || !h || !b || !lamW || !lamh || !lamb || !gW || !gh || !gb || !gpar || n < 1)
+        return fail(E_ARG, "invalid argument");
+    int rc;
+    if ((rc = batch_uniform(B)) || (rc = batch_pull(B))) return rc;
+    const int ni = B->n;
+    const int nx = B->inst[0]->D.nx;
+    const size_t nc = B->inst[0]->D.ncell, cv = nc * B->inst[0]->D.nvel;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    if (B->dseg_cap < n * ni) {
+        if (B->dseg) hipFree(B->dseg);
+        B->dseg = nullptr;
+        B->dseg_cap = 0;
+        if (hipMalloc(&B->dseg, sizeof(Dev) * n * ni) != hipSuccess) return fail(E_DEVICE, "allocation failed");
+        B->dseg_cap = n * ni;
+    }
+    for (int i = 0; i < ni; ++i) {
+        Instance* I = B->inst[i];
+        if ((rc = cn_buffers(I)) || (rc = adj_buffers(I)) || (rc = seg_buffers(I, n))) return rc;
+        HIPCHK(hipStreamSynchronize(I->stream));
+        Dev& D = I->D;
+        HIPCHK(hipMemcpyAsync(D.W, W + 4 * nc * i, sizeof(double) * 4 * nc, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(D.h, h + cv * i, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(D.b, b + cv * i, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(D.a_lamW, lamW + 4 * nc * i, sizeof(double) * 4 * nc, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(D.a_lamh, lamh + cv * i, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(D.a_lamb, lamb + cv * i, sizeof(double) * cv, dd, B->stream));
+        if (gwall)
+            HIPCHK(hipMemcpyAsync(I->dgwall, gwall + (size_t)i * n * 8 * nx, sizeof(double) * 8 * nx * n,
+                                  hipMemcpyHostToDevice, B->stream));
+        else
+            HIPCHK(hipMemsetAsync(I->dgwall, 0, sizeof(double) * 8 * nx * n, B->stream));
+        I->operator_fresh = false;
+        I->cn_targets_fresh = false;
+    }
+    std::vector<Dev> hd((size_t)n * ni);
+    for (int k = 0; k < n; ++k)
+        for (int i = 0; i < ni; ++i) hd[(size_t)k * ni + i] = seg_desc(B->inst[i], k, n);
+    HIPCHK(hipMemcpyAsync(B->dseg, hd.data(), sizeof(Dev) * n * ni, hipMemcpyHostToDevice, B->stream));
+    for (int k = 0; k < n; ++k) {
+        LaunchSet L = {&hd[(size_t)k * ni], B->dseg + (size_t)k * ni, ni};
+        HIPCHK(ugks_launch_cn(&L, k == 0 ? 1 : 0, B->stream, nullptr));
+    }
+    if ((rc = batch_status(B))) return rc;
+    for (int k = n - 1; k >= 0; --k) {
+        LaunchSet L = {&hd[(size_t)k * ni], B->dseg + (size_t)k * ni, ni};
+        HIPCHK(ugks_launch_cn_adjoint(&L, B->stream, nullptr));
+    }
+    std::vector<double> sums((size_t)4 * n * ni);
+    for (int i = 0; i < ni; ++i) {
+        const Dev& d0 = hd[i];
+        HIPCHK(hipMemcpyAsync(gW + 4 * nc * i, d0.a_Wout, sizeof(double) * 4 * nc, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(gh + cv * i, d0.a_h, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(gb + cv * i, d0.a_b, sizeof(double) * cv, dd, B->stream));
+        HIPCHK(hipMemcpyAsync(sums.data() + (size_t)4 * n * i, B->inst[i]->dsums, sizeof(double) * 4 * n,
+                              hipMemcpyDeviceToHost, B->stream));
+    }
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (int i = 0; i < ni; ++i)
+        for (int m = 0; m < 4; ++m) {
+            double t = 0.0;
+            for (int k = 0; k < n; ++k) t += sums[(size_t)4 * n * i + 4 * k + m];
+            gpar[4 * i + m] = t;
+        }
+    B->device_truth = false;
+    return batch_push(B);
+}
+
+// Wall speeds and mu_ref of batch member i (the batch's device descriptors
+// are updated with it).
+extern "C" int ugks_batch_set_physics(void* batch, int i, double mu_ref, double wall_ux_bottom, double wall_ux_top) {
+    Batch* B = (Batch*)batch;
+    if (!B || i < 0 || i >= B->n) return fail(E_ARG, "invalid batch member");
+    int rc;
+    if ((rc = batch_pull(B))) return rc;
+    if ((rc = ugks_set_physics(B->inst[i], mu_ref, wall_ux_bottom, wall_ux_top))) return rc;
+    B->device_truth = false;
+    return batch_push(B);
 }
