@@ -89,6 +89,32 @@ int ugks_norms(void* handle, double* out24, double* walls16);
  * moment_error, M_before, M_after, M0, dt, dtp, status bits, ...]. */
 int ugks_implicit_step(void* handle, int* accepted, double* diag16);
 
+/* Restart scalars of the closed-Couette iteration (what a complete restart
+ * carries besides the state).
+ * ugks_set_mass_target: the persistent mass target M0 and correction ledger
+ *   (4) of closed_mass_rank*.bin, replacing ReadMassRestart's load
+ *   (src/mod_ClosedCouetteMass.f90:348-393); without it a run takes M0 from
+ *   its first state, as after InitFlow_Freestream (:212-241).
+ * ugks_get_mass_target: out5 = M0 (NaN until defined) and the ledger, the
+ *   values WriteMassRestart stores (:325-346).
+ * ugks_inherit_references: the residual normalisation epoch of
+ *   residual_reference.in (StartR5Control, src/mod_R5Observe.f90:176-187);
+ *   call before the first evaluation of the restarted state.
+ * ugks_get_references: GetImplicitReferences (reference_h, reference_b), NaN
+ *   before the first evaluation (WriteRestartReference, src/mod_Output.f90:58-76).
+ * ugks_cold_start resets all three (a new run). */
+int ugks_set_mass_target(void* handle, double m0, const double* ledger4);
+int ugks_get_mass_target(void* handle, double* out5);
+int ugks_inherit_references(void* handle, double ref_h, double ref_b);
+int ugks_get_references(void* handle, double* out2);
+
+/* MassStatistics of the current state (src/mod_ClosedCouetteMass.f90:173-204):
+ * totals5 = integral of W(1:4) and of the discrete density of h; metrics5 =
+ * max moment error, max negative fraction of h and of b, min rho, min internal
+ * energy; *ok = finite, admissible and moment error <= moment_tol.  The M_W,
+ * M_h and moment columns of sparse_residual.csv. */
+int ugks_mass_statistics(void* handle, double* totals5, double* metrics5, int* ok);
+
 /* One S2CN2 explicit step (the TimeIntegration_method = 'S2CN2' path):
  * the S2 fluxes of the current state (as in ugks_evolution), the streamed
  * state, volume targets at the streamed macro state and the Crank-Nicolson

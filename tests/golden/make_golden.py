@@ -17,6 +17,13 @@ Run once in a container that holds the reference checkout (default
                          (science_*, flux_*, R*, c_minus_1), and the printed
                          case constants (stdout.log).
 
+  formats/               - data files of the reference's runs that pin the
+                         on-disk and monitor formats byte for byte:
+                         Mesh2d.dat (a Fortran sequential file of the shipped
+                         8x64 mesh), the release runs' sparse_residual.csv and
+                         R5_status.txt (cases/high_implicit, low_implicit) and
+                         their "Implicit iter=" stdout lines.
+
 Nothing from the reference's source code is copied; the fixtures are data.
 """
 from __future__ import annotations
@@ -79,5 +86,27 @@ def main():
     print("explicit release:", ex)
 
 
+def formats():
+    out = os.path.join(HERE, "formats")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(REF, "cases/high_implicit/Mesh2d.dat"), "rb") as f:
+        data = f.read()
+    with open(os.path.join(out, "Mesh2d.dat"), "wb") as f:
+        f.write(data)
+    for run in ("high_implicit", "low_implicit"):
+        base = os.path.join(REF, "docs/release_checks", run)
+        for name in ("sparse_residual.csv", "R5_status.txt"):
+            with open(os.path.join(base, name), "rb") as f:
+                data = f.read()
+            with open(os.path.join(out, f"{run}_{name}"), "wb") as f:
+                f.write(data)
+        with open(os.path.join(base, "stdout.log")) as f:
+            lines = [l for l in f if l.startswith("Implicit iter=")]
+        with open(os.path.join(out, f"{run}_implicit_iter.txt"), "w") as f:
+            f.writelines(lines)
+    print("format fixtures in", out)
+
+
 if __name__ == "__main__":
     main()
+    formats()

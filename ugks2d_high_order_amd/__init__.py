@@ -27,7 +27,8 @@ EXPORTS = [
     "ugks_batch_create", "ugks_batch_destroy", "ugks_batch_instance", "ugks_batch_cold_start", "ugks_batch_run",
     "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
     "ugks_cn_adjoint_device", "ugks_cn_adjoint_segment", "ugks_set_physics", "ugks_batch_cn_forward",
-    "ugks_batch_cn_adjoint_segment", "ugks_batch_set_physics",
+    "ugks_batch_cn_adjoint_segment", "ugks_batch_set_physics", "ugks_set_mass_target", "ugks_get_mass_target",
+    "ugks_inherit_references", "ugks_get_references", "ugks_mass_statistics",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -98,6 +99,11 @@ def load_library(path: str = LIB_PATH):
     L.ugks_batch_set_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.ugks_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.ugks_observables.argtypes = [C.c_void_p, _D]
+    L.ugks_set_mass_target.argtypes = [C.c_void_p, C.c_double, _D]
+    L.ugks_get_mass_target.argtypes = [C.c_void_p, _D]
+    L.ugks_inherit_references.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.ugks_get_references.argtypes = [C.c_void_p, _D]
+    L.ugks_mass_statistics.argtypes = [C.c_void_p, _D, _D, C.POINTER(C.c_int)]
     L.ugks_sync.argtypes = [C.c_void_p]
     L.ugks_debug_fetch.argtypes = [C.c_void_p, C.c_int, _D, C.c_long]
     L.ugks_profile.argtypes = [C.c_void_p, C.c_int]
@@ -279,6 +285,37 @@ class UGKS:
         """Reference viscosity and wall speeds from the next step on."""
         _check(load_library().ugks_set_physics(self.handle, float(mu_ref), float(wall_ux_bottom),
                                                float(wall_ux_top)))
+
+    def set_mass_target(self, m0: float, ledger=(0.0, 0.0, 0.0, 0.0)):
+        """Persistent mass target M0 and correction ledger of a complete
+        restart (ReadMassRestart, src/mod_ClosedCouetteMass.f90:348-393)."""
+        led = np.ascontiguousarray(ledger, dtype=np.float64).reshape(4)
+        _check(load_library().ugks_set_mass_target(self.handle, float(m0), _ptr(led)))
+
+    def mass_target(self):
+        """(M0, ledger[4]); M0 is NaN until the first update defines it."""
+        o = np.zeros(5)
+        _check(load_library().ugks_get_mass_target(self.handle, _ptr(o)))
+        return float(o[0]), o[1:].copy()
+
+    def inherit_references(self, ref_h: float, ref_b: float):
+        """Residual normalisation epoch of residual_reference.in
+        (InheritImplicitReferences, src/mod_R5Observe.f90:176-187)."""
+        _check(load_library().ugks_inherit_references(self.handle, float(ref_h), float(ref_b)))
+
+    def references(self):
+        """GetImplicitReferences: (reference_h, reference_b), NaN before the
+        first evaluation."""
+        o = np.zeros(2)
+        _check(load_library().ugks_get_references(self.handle, _ptr(o)))
+        return float(o[0]), float(o[1])
+
+    def mass_statistics(self):
+        """MassStatistics (src/mod_ClosedCouetteMass.f90:173-204) of the
+        current state: totals[5], metrics[5], ok."""
+        t = np.zeros(5); m = np.zeros(5); ok = C.c_int(0)
+        _check(load_library().ugks_mass_statistics(self.handle, _ptr(t), _ptr(m), C.byref(ok)))
+        return t, m, bool(ok.value)
 
     def run(self, n: int) -> int:
         done = C.c_int(0)

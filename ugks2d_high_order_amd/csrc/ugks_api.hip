@@ -29,6 +29,7 @@ extern "C" hipError_t ugks_launch_cold_start(const LaunchSet* L, double rho, dou
 extern "C" hipError_t ugks_launch_observables(const LaunchSet* L, double* out, hipStream_t s);
 extern "C" hipError_t ugks_launch_lo_step(const LaunchSet* L, int wg, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_lo_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s);
+extern "C" hipError_t ugks_launch_mass_stats(const LaunchSet* L, double* part, double moment_tol, hipStream_t s);
 
 namespace {
 
@@ -73,6 +74,7 @@ struct Instance {
     double* dsums = nullptr;      // [step][4] parameter gradients
     int seg_cap = 0;
     int lo_wg = 256;              // workgroup size of the low-order Explicit kernels
+    double* mstat = nullptr;      // [cell][12] partials of the mass statistics (first use)
 };
 
 // Polynomial coefficients (degrees 0..n-1) whose averages over the unit cells
@@ -308,6 +310,18 @@ extern "C" void ugks_destroy(void* handle) {
     delete I;
 }
 
+// read-modify-write of the device scalars (ordered on the instance stream)
+template <typename F>
+static int scal_edit(Instance* I, F edit) {
+    double S[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(S, I->D.scal, sizeof S, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    edit(S);
+    HIPCHK(hipMemcpyAsync(I->D.scal, S, sizeof S, hipMemcpyHostToDevice, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return E_OK;
+}
+
 // a new state starts a new S2CN2 run: its targets and conservation ledger
 static int reset_cn(Instance* I) {
     static const double zero = 0.0;
@@ -364,7 +378,18 @@ extern "C" int ugks_cold_start(void* handle, double rho, double ux, double t_nat
     HIPCHK(ugks_launch_cold_start(&L, rho, ux, t_native, I->stream));
     I->operator_fresh = false;
     int rc = check_status(I, nullptr);
-    return rc ? rc : reset_cn(I);
+    if (rc) return rc;
+    // a cold start is a new run (InitFlow_Freestream + InitializeClosedMass,
+    // src/mod_ClosedCouetteMass.f90:212-241): the mass target is the mass of
+    // this state, the ledger starts at zero and the residual references come
+    // from its first evaluation
+    if ((rc = scal_edit(I, [](double* S) {
+             S[SC_REF_INIT] = 0.0;
+             S[SC_M0_INIT] = 0.0;
+             for (int a = 0; a < 4; ++a) S[SC_LEDGER + a] = S[SC_LEDGER_NEXT + a] = 0.0;
+         })))
+        return rc;
+    return reset_cn(I);
 }
 
 extern "C" int ugks_sync(void* handle) {
@@ -438,10 +463,95 @@ extern "C" int ugks_implicit_step(void* handle, int* accepted, double* diag16) {
     std::swap(I->D.W, I->D.Wn);
     std::swap(I->D.h, I->D.hn);
     std::swap(I->D.b, I->D.bn);
+    HIPCHK(hipMemcpyAsync(I->D.scal + SC_LEDGER, I->D.scal + SC_LEDGER_NEXT, 4 * sizeof(double),
+                          hipMemcpyDeviceToDevice, I->stream));
     if ((rc = upload(I))) return rc;
     I->operator_fresh = false;
     I->cn_targets_fresh = false;
     *accepted = 1;
+    return E_OK;
+}
+
+// Restart scalars of the closed-Couette iteration.  The mass target M0 and
+// the correction ledger a complete restart carries (ReadMassRestart,
+// src/mod_ClosedCouetteMass.f90:348-393): the target replaces the one a run
+// would take from its first state.
+extern "C" int ugks_set_mass_target(void* handle, double m0, const double* ledger4) {
+    Instance* I = (Instance*)handle;
+    if (!I || !ledger4) return fail(E_ARG, "null argument");
+    if (!(std::isfinite(m0) && m0 > 0.0)) return fail(E_ARG, "invalid mass target M0");
+    for (int a = 0; a < 4; ++a)
+        if (!std::isfinite(ledger4[a])) return fail(E_ARG, "non-finite correction ledger");
+    return scal_edit(I, [&](double* S) {
+        S[SC_M0] = m0;
+        S[SC_M0_INIT] = 1.0;
+        for (int a = 0; a < 4; ++a) S[SC_LEDGER + a] = S[SC_LEDGER_NEXT + a] = ledger4[a];
+    });
+}
+
+// out5 = M0 (NaN until the first update or ugks_set_mass_target defines it)
+// and the correction ledger (4)
+extern "C" int ugks_get_mass_target(void* handle, double* out5) {
+    Instance* I = (Instance*)handle;
+    if (!I || !out5) return fail(E_ARG, "null argument");
+    double S[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(S, I->D.scal, sizeof S, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    out5[0] = S[SC_M0_INIT] != 0.0 ? S[SC_M0] : std::nan("");
+    for (int a = 0; a < 4; ++a) out5[1 + a] = S[SC_LEDGER + a];
+    return E_OK;
+}
+
+// Residual normalisation epoch inherited from residual_reference.in
+// (StartR5Control -> InheritImplicitReferences, src/mod_R5Observe.f90:176-187):
+// the next evaluations are normalised by these raw h/b norms instead of
+// their own first evaluation's.
+extern "C" int ugks_inherit_references(void* handle, double ref_h, double ref_b) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    if (!(std::isfinite(ref_h) && std::isfinite(ref_b) && ref_h >= 0.0 && ref_b >= 0.0))
+        return fail(E_ARG, "invalid residual references");
+    int rc = scal_edit(I, [&](double* S) {
+        S[SC_REF_H] = ref_h;
+        S[SC_REF_B] = ref_b;
+        S[SC_REF_INIT] = 1.0;
+    });
+    I->operator_fresh = false;
+    return rc;
+}
+
+// GetImplicitReferences: out2 = (reference_h, reference_b), NaN before the
+// first evaluation of a run
+extern "C" int ugks_get_references(void* handle, double* out2) {
+    Instance* I = (Instance*)handle;
+    if (!I || !out2) return fail(E_ARG, "null argument");
+    double S[SC_COUNT];
+    HIPCHK(hipMemcpyAsync(S, I->D.scal, sizeof S, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    const bool set = S[SC_REF_INIT] != 0.0;
+    out2[0] = set ? S[SC_REF_H] : std::nan("");
+    out2[1] = set ? S[SC_REF_B] : std::nan("");
+    return E_OK;
+}
+
+// MassStatistics of the current state (src/mod_ClosedCouetteMass.f90:173-204):
+// totals5 = integral W(1:4) and integral of the discrete density of h,
+// metrics5 = max moment error, max negative fraction of h and of b, min rho,
+// min internal energy; *ok = the reference's gate (finite, admissible,
+// moment error <= Implicit_MomentTol).
+extern "C" int ugks_mass_statistics(void* handle, double* totals5, double* metrics5, int* ok) {
+    Instance* I = (Instance*)handle;
+    if (!I) return fail(E_ARG, "null handle");
+    int rc;
+    if (!I->mstat && (rc = dmalloc(I, &I->mstat, (size_t)I->D.ncell * 12))) return rc;
+    LaunchSet L = one(I);
+    HIPCHK(ugks_launch_mass_stats(&L, I->mstat, I->p.moment_tol, I->stream));
+    double S[11];
+    HIPCHK(hipMemcpyAsync(S, I->D.scal + SC_MSTAT, sizeof S, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    if (totals5) for (int a = 0; a < 5; ++a) totals5[a] = S[a];
+    if (metrics5) for (int a = 0; a < 5; ++a) metrics5[a] = S[5 + a];
+    if (ok) *ok = S[10] != 0.0 ? 1 : 0;
     return E_OK;
 }
 

@@ -539,18 +539,19 @@ __global__ void __launch_bounds__(256) k_recover(const Dev* __restrict__ DS) {
 
 __global__ void __launch_bounds__(256) k_mass(const Dev* __restrict__ DS, int rescale) {
     const Dev& D = DS[blockIdx.z];
-    __shared__ double lds[16 * 1];
-    double s[1] = {0.0}, s0[1] = {0.0};
+    __shared__ double lds[16 * 4];
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, s0[1] = {0.0};
     const double area = D.dx * D.dy;
     for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
-        s[0] += area * D.Wn[4 * c];
+        for (int a = 0; a < 4; ++a) s[a] += area * D.Wn[4 * c + a];
         s0[0] += area * D.W[4 * c];
     }
-    block_sum<1>(s, lds);
+    block_sum<4>(s, lds);
     block_sum<1>(s0, lds);
     if (threadIdx.x == 0) {
         if (D.scal[SC_M0_INIT] == 0.0) { D.scal[SC_M0] = s0[0]; D.scal[SC_M0_INIT] = 1.0; }
         D.scal[SC_MASS_BEFORE] = s[0];
+        for (int a = 0; a < 4; ++a) D.scal[SC_MB4 + a] = s[a];
         D.scal[SC_SCALE] = rescale ? D.scal[SC_M0] / s[0] : 1.0;
     }
 }
@@ -568,13 +569,102 @@ __global__ void __launch_bounds__(256) k_rescale(const Dev* __restrict__ DS) {
         D.Wn[t] = c * D.Wn[t];
 }
 
+// totals after the rescale; the correction ledger the proposal would carry
+// (correction_ledger + mass_after(1:4) - mass_before(1:4),
+// src/mod_ClosedCouetteMass.f90:284-310), committed on acceptance
 __global__ void __launch_bounds__(256) k_mass_after(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
-    __shared__ double lds[16 * 1];
-    double s[1] = {0.0};
-    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) s[0] += D.dx * D.dy * D.Wn[4 * c];
-    block_sum<1>(s, lds);
-    if (threadIdx.x == 0) D.scal[SC_MASS_AFTER] = s[0];
+    __shared__ double lds[16 * 4];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x)
+        for (int a = 0; a < 4; ++a) s[a] += D.dx * D.dy * D.Wn[4 * c + a];
+    block_sum<4>(s, lds);
+    if (threadIdx.x == 0) {
+        D.scal[SC_MASS_AFTER] = s[0];
+        for (int a = 0; a < 4; ++a) D.scal[SC_LEDGER_NEXT + a] = D.scal[SC_LEDGER + a] + (s[a] - D.scal[SC_MB4 + a]);
+    }
+}
+
+// ===========================================================================
+// MassStatistics of the accepted state (src/mod_ClosedCouetteMass.f90:173-204)
+// ===========================================================================
+// Per cell (one workgroup): the discrete moments mm of (h, b)
+// (MomentConserved, src/mod_ImplicitCommon.f90:32-40), the moment error
+// max|W - mm| / max(|W|, 1), the negative-mass fractions of h and b, rho and
+// the internal energy, and the cell's admissibility.  part[c*12 + ...]:
+// A*W(4), A*mm(1), moment error, nh, nb, rho (or +inf), e (or +inf), ok.
+__global__ void __launch_bounds__(256) k_mass_stats_cell(const Dev* __restrict__ DS, double* __restrict__ part) {
+    const Dev& D = DS[blockIdx.z];
+    const int c = blockIdx.x;
+    const int nv = D.nvel;
+    __shared__ double lds[16 * 9];
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double* h = D.h + (size_t)c * nv;
+    const double* b = D.b + (size_t)c * nv;
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        const double w = D.w[k], u = D.u[k], v = D.v[k], hk = h[k], bk = b[k];
+        s[0] += w * hk;
+        s[1] += w * u * hk;
+        s[2] += w * v * hk;
+        s[3] += w * ((u * u + v * v) * hk + bk);
+        s[4] += w * fmax(-hk, 0.0);
+        s[5] += w * fabs(hk);
+        s[6] += w * fmax(-bk, 0.0);
+        s[7] += w * fabs(bk);
+        s[8] += (isfinite(hk) && isfinite(bk)) ? 0.0 : 1.0;
+    }
+    block_sum<9>(s, lds);
+    if (threadIdx.x != 0) return;
+    const double area = D.dx * D.dy;
+    double W[4], mm[4] = {s[0], s[1], s[2], 0.5 * s[3]};
+    for (int a = 0; a < 4; ++a) W[a] = D.W[4 * (size_t)c + a];
+    double err = 0.0;
+    for (int a = 0; a < 4; ++a) err = fmax(err, fabs(W[a] - mm[a]) / fmax(fabs(W[a]), 1.0));
+    double* o = part + (size_t)c * 12;
+    for (int a = 0; a < 4; ++a) o[a] = area * W[a];
+    o[4] = area * mm[0];
+    o[5] = err;
+    o[6] = s[4] / fmax(s[5], UGKS_DBL_MIN);
+    o[7] = s[6] / fmax(s[7], UGKS_DBL_MIN);
+    const bool valid = isfinite(W[0]) && W[0] > 0.0;
+    o[8] = valid ? W[0] : __builtin_inf();
+    o[9] = valid ? W[3] - 0.5 * (W[1] * W[1] + W[2] * W[2]) / W[0] : __builtin_inf();
+    o[10] = (s[8] == 0.0 && physical_w(W) && physical_w(mm) && area > 0.0) ? 1.0 : 0.0;
+}
+
+// totals(5), metrics(5) and ok into scal[SC_MSTAT..SC_MSTAT+10] (one workgroup)
+__global__ void __launch_bounds__(256) k_mass_stats_reduce(const Dev* __restrict__ DS, const double* __restrict__ part,
+                                                             double moment_tol) {
+    const Dev& D = DS[blockIdx.z];
+    __shared__ double lds[16 * 5];
+    double t[5] = {0, 0, 0, 0, 0}, mx[5] = {0.0, 0.0, 0.0, -__builtin_inf(), -__builtin_inf()}, bad[1] = {0.0};
+    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x) {
+        const double* o = part + (size_t)c * 12;
+        for (int a = 0; a < 5; ++a) t[a] += o[a];
+        mx[0] = fmax(mx[0], o[5]);
+        mx[1] = fmax(mx[1], o[6]);
+        mx[2] = fmax(mx[2], o[7]);
+        mx[3] = fmax(mx[3], -o[8]);
+        mx[4] = fmax(mx[4], -o[9]);
+        bad[0] += 1.0 - o[10];
+    }
+    block_sum<5>(t, lds);
+    block_max<5>(mx, lds);
+    block_sum<1>(bad, lds);
+    if (threadIdx.x != 0) return;
+    double* S = D.scal + SC_MSTAT;
+    bool ok = bad[0] == 0.0;
+    for (int a = 0; a < 5; ++a) { S[a] = t[a]; ok = ok && isfinite(t[a]); }
+    S[5] = mx[0]; S[6] = mx[1]; S[7] = mx[2]; S[8] = -mx[3]; S[9] = -mx[4];
+    for (int a = 5; a < 10; ++a) ok = ok && isfinite(S[a]);
+    S[10] = (ok && S[5] <= moment_tol) ? 1.0 : 0.0;
+}
+
+extern "C" hipError_t ugks_launch_mass_stats(const LaunchSet* L, double* part, double moment_tol, hipStream_t s) {
+    const Dev& D = *L->size;
+    hipLaunchKernelGGL(k_mass_stats_cell, dim3(D.ncell, 1, L->n), dim3(256), 0, s, L->dev, part);
+    hipLaunchKernelGGL(k_mass_stats_reduce, dim3(1, 1, L->n), dim3(256), 0, s, L->dev, part, moment_tol);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t ugks_launch_wall_reflection(const LaunchSet* L, hipStream_t s, Prof* P) {
@@ -634,6 +724,7 @@ __global__ void __launch_bounds__(64) k_batch_accept(Dev* DS, int n, int* count,
         t = D.W; D.W = D.Wn; D.Wn = t;
         t = D.h; D.h = D.hn; D.hn = t;
         t = D.b; D.b = D.bn; D.bn = t;
+        for (int a = 0; a < 4; ++a) D.scal[SC_LEDGER + a] = D.scal[SC_LEDGER_NEXT + a];
         count[z] += 1;
     }
 }

@@ -23,7 +23,11 @@ enum ScalarSlot {
     SC_CN_BAL = 68,     // 4 cumulative wall balance (top - bottom, per unit length / dy)
     SC_CN_DIAG = 72,    // 16 diagnostics of the last S2CN2 step
     SC_LO_RES = 88,     // residual res(1:4) of the last low-order Explicit step
-    SC_COUNT = 96
+    SC_LEDGER = 92,     // 4 closed-Couette correction ledger: sum of (after - before) totals of accepted rescales
+    SC_MB4 = 96,        // 4 totals sum A*W of the proposal before the rescale
+    SC_LEDGER_NEXT = 100,  // 4 ledger if the current proposal is accepted
+    SC_MSTAT = 104,     // 11 mass statistics of the accepted state: totals(5), metrics(5), ok
+    SC_COUNT = 120
 };
 
 // Tapes of the S2CN2 forward kept for the reverse step (written only when

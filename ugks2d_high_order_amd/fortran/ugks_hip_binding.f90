@@ -108,6 +108,40 @@ module ugks_hip
       type(c_ptr), value :: handle
       real(c_double), value :: rho, ux, t_native
     end function ugks_explicit_cold_start
+
+    ! restart scalars: M0 + correction ledger (closed_mass_rank*.bin) and the
+    ! residual epoch (residual_reference.in)
+    integer(c_int) function ugks_set_mass_target(handle, m0, ledger4) bind(C, name="ugks_set_mass_target")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), value :: m0
+      real(c_double), intent(in) :: ledger4(4)
+    end function ugks_set_mass_target
+
+    integer(c_int) function ugks_get_mass_target(handle, out5) bind(C, name="ugks_get_mass_target")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: out5(5)
+    end function ugks_get_mass_target
+
+    integer(c_int) function ugks_inherit_references(handle, ref_h, ref_b) bind(C, name="ugks_inherit_references")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), value :: ref_h, ref_b
+    end function ugks_inherit_references
+
+    integer(c_int) function ugks_get_references(handle, out2) bind(C, name="ugks_get_references")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: out2(2)
+    end function ugks_get_references
+
+    integer(c_int) function ugks_mass_statistics(handle, totals5, metrics5, ok) bind(C, name="ugks_mass_statistics")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: totals5(5), metrics5(5)
+      integer(c_int), intent(out) :: ok
+    end function ugks_mass_statistics
   end interface
 
 contains
