@@ -74,6 +74,14 @@ inline double __shfl_xor(double x, int mask, int width = 64) {
     unsigned lane = threadIdx.x % width, base = threadIdx.x - lane;
     return B.xbuf[base + (lane ^ (unsigned)mask)];
 }
+inline double __shfl(double x, int src, int width = 64) {
+    EmuBlock& B = *emu_block;
+    B.bar.arrive_and_wait();
+    B.xbuf[threadIdx.x] = x;
+    B.bar.arrive_and_wait();
+    unsigned lane = threadIdx.x % width, base = threadIdx.x - lane;
+    return B.xbuf[base + ((unsigned)src % (unsigned)width)];
+}
 inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 
 // ---- runtime API --------------------------------------------------------

@@ -131,3 +131,23 @@ def test_shipped_case_initial_norms_and_two_steps(oracle_mod, release):
         e = (_nrel(Wg, Wo), _nrel(hg, ho), _nrel(bg, bo))
         print("shipped iter", it + 1, e, flush=True)
         assert max(e) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 6), (72, 6)])
+def test_pr2_paths_match_oracle(oracle_mod, nx, ny):
+    """PR(2) runs in one wavefront (lane per column) up to 64 columns and in
+    one workgroup beyond; both follow the oracle's hyperplane Gauss-Seidel."""
+    case = config.small_case(nx=nx, ny=ny)
+    g, o = _pair(case, oracle_mod)
+    o.cold_start(1.0, 0.0, case.t_init)
+    g.set_state(*o.state())
+    print("pr2 path", nx, "x", ny, flush=True)
+    for it in range(3):
+        acc_g, dg = g.implicit_step()
+        st, acc_o, do = o.iteration()
+        assert acc_g and acc_o and st == 0
+        Wg, hg, bg = g.state()
+        Wo, ho, bo = o.state()
+        e = (_nrel(Wg, Wo), _nrel(hg, ho), _nrel(bg, bo))
+        print("iter", it + 1, e, flush=True)
+        assert max(e) < 1e-12, (it, e)

@@ -77,61 +77,84 @@ __global__ void __launch_bounds__(256) k_macro_gamma(const Dev* __restrict__ DS)
     }
 }
 
-// RelaxMacroPoint of cell (i,j); returns false on a non-finite result
-__device__ inline bool relax_macro(const Dev& D, int i, int j, double dtp) {
-    const int nx = D.nx, ny = D.ny, c = j * nx + i;
-    double wi[4], di[4];
-    for (int m = 0; m < 4; ++m) { wi[m] = D.W[4 * c + m]; di[m] = D.dW[4 * c + m]; }
+// RelaxMacroPoint arithmetic of one cell (src/mod_ImplicitMacro.f90:15-263):
+// wi, di its state and current increment; per face f (west, east, south,
+// north) the neighbour state wn[f], its increment dn[f] and the spectral
+// radius g[f] (wall faces carry the Zhu images of wi and di); rate its
+// residual rate.  Writes the new increment into out and returns false on a
+// non-finite result.
+__device__ __attribute__((always_inline)) inline bool relax_core(const Dev& D, const double wi[4], const double di[4], const double wn[4][4],
+                                  const double dn[4][4], const double g[4], const double rate[4], double dtp,
+                                  double out[4]) {
     double sdt[4] = {0, 0, 0, 0}, sgn[4] = {0, 0, 0, 0}, sg = 0.0;
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
-        double nxo = 0.0, nyo = 0.0, len, g, wj[4], dn[4];
-        bool wall = false;
-        int top = 0;
-        if (f == 0) { nxo = -1.0; len = D.dy; }
-        else if (f == 1) { nxo = 1.0; len = D.dy; }
-        else if (f == 2) { nyo = -1.0; len = D.dx; wall = j == 0; }
-        else { nyo = 1.0; len = D.dx; wall = j == ny - 1; top = 1; }
-        if (wall) {
-            double uw = D.wall_ux[top];
-            zhu_map(wi, uw, 0.0, wj);
-            zhu_map(di, uw, 0.0, dn);
-            g = D.gy[(size_t)(top ? ny : 0) * nx + i];
-        } else {
-            int ni = i, nj = j;
-            bool halo = false;
-            if (f == 0) { ni = i - 1; if (ni < 0) { ni = nx - 1; halo = true; } g = D.gx[j * nx + ni]; }
-            else if (f == 1) { ni = i + 1; if (ni == nx) { ni = 0; halo = true; } g = D.gx[c]; }
-            else if (f == 2) { nj = j - 1; g = D.gy[(size_t)j * nx + i]; }
-            else { nj = j + 1; g = D.gy[(size_t)(j + 1) * nx + i]; }
-            int cn = nj * nx + ni;
-            const double* src = halo ? D.dWs : D.dW;
-            for (int m = 0; m < 4; ++m) { wj[m] = D.W[4 * cn + m]; dn[m] = src[4 * cn + m]; }
-        }
+        const double nxo = f == 0 ? -1.0 : (f == 1 ? 1.0 : 0.0);
+        const double nyo = f == 2 ? -1.0 : (f == 3 ? 1.0 : 0.0);
+        const double len = f < 2 ? D.dy : D.dx;
         double a[4], fnew[4], fold[4], ti[4], tj[4];
         for (int m = 0; m < 4; ++m) a[m] = wi[m] + di[m];
         euler_flux(D.gas.gamma, a, nxo, nyo, fnew);
         euler_flux(D.gas.gamma, wi, nxo, nyo, fold);
         for (int m = 0; m < 4; ++m) ti[m] = fnew[m] - fold[m];
-        for (int m = 0; m < 4; ++m) a[m] = wj[m] + dn[m];
+        for (int m = 0; m < 4; ++m) a[m] = wn[f][m] + dn[f][m];
         euler_flux(D.gas.gamma, a, nxo, nyo, fnew);
-        euler_flux(D.gas.gamma, wj, nxo, nyo, fold);
+        euler_flux(D.gas.gamma, wn[f], nxo, nyo, fold);
         for (int m = 0; m < 4; ++m) tj[m] = fnew[m] - fold[m];
         for (int m = 0; m < 4; ++m) {
             sdt[m] += len * (ti[m] + tj[m]);
-            sgn[m] += len * g * dn[m];
+            sgn[m] += len * g[f] * dn[f][m];
         }
-        sg += len * g;
+        sg += len * g[f];
     }
     const double area = D.dx * D.dy;
     double diag = 1.0 / dtp + 0.5 * sg / area;
-    double rhs[4];
     bool ok = isfinite(diag) && diag > 0.0;
+    double rhs[4];
     for (int m = 0; m < 4; ++m) {
-        rhs[m] = D.rate_w[4 * c + m] - 0.5 * sdt[m] / area + 0.5 * sgn[m] / area;
+        rhs[m] = rate[m] - 0.5 * sdt[m] / area + 0.5 * sgn[m] / area;
         ok = ok && isfinite(rhs[m]);
     }
+    for (int m = 0; m < 4; ++m) out[m] = rhs[m] / diag;
+    return ok;
+}
+
+// Neighbour data of face f of cell (i,j) read from memory: the neighbour's
+// state, its increment from `src` (the seam halo reads the copy of the last
+// sweep) and the face's spectral radius; wall faces take the Zhu images.
+__device__ __attribute__((always_inline)) inline void face_inputs(const Dev& D, int i, int j, int f, const double wi[4], const double di[4],
+                                   double wn[4], double dn[4], double* g) {
+    const int nx = D.nx, ny = D.ny, c = j * nx + i;
+    const bool wall = (f == 2 && j == 0) || (f == 3 && j == ny - 1);
+    if (wall) {
+        const int top = f == 3;
+        const double uw = D.wall_ux[top];
+        zhu_map(wi, uw, 0.0, wn);
+        zhu_map(di, uw, 0.0, dn);
+        *g = D.gy[(size_t)(top ? ny : 0) * nx + i];
+        return;
+    }
+    int ni = i, nj = j;
+    bool halo = false;
+    if (f == 0) { ni = i - 1; if (ni < 0) { ni = nx - 1; halo = true; } *g = D.gx[j * nx + ni]; }
+    else if (f == 1) { ni = i + 1; if (ni == nx) { ni = 0; halo = true; } *g = D.gx[c]; }
+    else if (f == 2) { nj = j - 1; *g = D.gy[(size_t)j * nx + i]; }
+    else { nj = j + 1; *g = D.gy[(size_t)(j + 1) * nx + i]; }
+    const int cn = nj * nx + ni;
+    const double* src = halo ? D.dWs : D.dW;
+    for (int m = 0; m < 4; ++m) { wn[m] = D.W[4 * cn + m]; dn[m] = src[4 * cn + m]; }
+}
+
+// RelaxMacroPoint of cell (i,j) with every input read from memory; returns
+// false on a non-finite result (the increment is then left as it was)
+__device__ inline bool relax_macro(const Dev& D, int i, int j, double dtp) {
+    const int c = j * D.nx + i;
+    double wi[4], di[4], rate[4], wn[4][4], dn[4][4], g[4], out[4];
+    for (int m = 0; m < 4; ++m) { wi[m] = D.W[4 * c + m]; di[m] = D.dW[4 * c + m]; rate[m] = D.rate_w[4 * c + m]; }
+    for (int f = 0; f < 4; ++f) face_inputs(D, i, j, f, wi, di, wn[f], dn[f], &g[f]);
+    const bool ok = relax_core(D, wi, di, wn, dn, g, rate, dtp, out);
     if (ok)
-        for (int m = 0; m < 4; ++m) D.dW[4 * c + m] = rhs[m] / diag;
+        for (int m = 0; m < 4; ++m) D.dW[4 * c + m] = out[m];
     return ok;
 }
 
@@ -176,6 +199,301 @@ __global__ void __launch_bounds__(256) k_macro_pr2(const Dev* __restrict__ DS) {
     }
 }
 
+// The same PR(2) sweeps in one wavefront when nx <= 64 (k_pr2_stage,
+// k_pr2_wave, k_pr2_finish).  Lane i owns column i and walks the diagonals.
+//  * Staging: everything the sweeps read or write lives, for the update, in a
+//    diagonal-major copy (entry (d = i + j, i) at d*64 + i, one array per
+//    component) inside the first-stage line buffer, which the operator has
+//    finished with.  On a diagonal the 64 lanes then touch consecutive
+//    entries instead of 64 different rows.
+//  * The flux differences of a face, F(w + dW; n) - F(w; n), are kept per cell
+//    for the x and y normals (the opposite normal is the exact negative), for
+//    the current increment and for the copy of the last sweep (the seam
+//    halo); a cell's differences are formed once, when its increment
+//    changes, instead of once per face and neighbour.
+//  * On diagonal d the upstream neighbours (west and south going forward,
+//    east and north going back) are the cells this lane and its neighbour
+//    lane relaxed on the previous diagonal: their increments and differences
+//    come from registers and a lane shift.  Everything else a cell reads is
+//    fixed during the half sweep and is loaded one diagonal ahead.
+// Per cell the sums run over the faces in the same order as relax_core; the
+// results agree with k_macro_pr2 to rounding (the two contract differently).
+enum {
+    PS_W = 0, PS_RATE = 4, PS_G = 8, PS_DW = 12, PS_DWS = 16, PS_DL = 20, PS_DLS = 28, PS_N = 36
+};
+
+__device__ inline size_t pr2_len(const Dev& D) { return (size_t)(D.nx + D.ny - 1) * 64; }
+
+// diagonal-major staging of the static inputs; zero increments and differences
+__global__ void __launch_bounds__(256) k_pr2_stage(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.ncell) return;
+    const int nx = D.nx, i = c % nx, j = c / nx;
+    const size_t N = pr2_len(D), e = (size_t)(i + j) * 64 + i;
+    double* S = D.lines;
+    for (int m = 0; m < 4; ++m) {
+        S[(PS_W + m) * N + e] = D.W[4 * c + m];
+        S[(PS_RATE + m) * N + e] = D.rate_w[4 * c + m];
+        S[(PS_DW + m) * N + e] = 0.0;
+        S[(PS_DWS + m) * N + e] = 0.0;
+    }
+    for (int q = 0; q < 8; ++q) { S[(PS_DL + q) * N + e] = 0.0; S[(PS_DLS + q) * N + e] = 0.0; }
+    S[(PS_G + 0) * N + e] = D.gx[j * nx + (i == 0 ? nx - 1 : i - 1)];
+    S[(PS_G + 1) * N + e] = D.gx[c];
+    S[(PS_G + 2) * N + e] = D.gy[(size_t)j * nx + i];
+    S[(PS_G + 3) * N + e] = D.gy[(size_t)(j + 1) * nx + i];
+}
+
+struct MacroIn {
+    double wi[4], di[4], rate[4], dl[8], tj[4][4], dn[4][4], g[4];
+};
+
+// global-memory doubles: the staging pointer comes out of the descriptor, so
+// without the address space the compiler emits flat accesses, whose
+// completion counts also gate the LDS lane shifts (every shift would then
+// wait for the loads issued one diagonal ahead)
+typedef __attribute__((address_space(1))) double gdouble;
+
+// the scalars the sweeps use, held in registers (a global store could alias
+// the descriptor and force them to be reloaded after every relaxation)
+struct Pr2Ctx {
+    int nx, ny;
+    double dx, dy, gamma, uw[2];
+};
+
+// inputs of cell (i, j) from the staging arrays
+__device__ __attribute__((always_inline)) inline void macro_load(const Pr2Ctx& D, const gdouble* __restrict__ S,
+                                                                 size_t N, int i, int j, MacroIn& q) {
+    const int nx = D.nx, ny = D.ny;
+    const double gam = D.gamma;
+    const size_t e = (size_t)(i + j) * 64 + i;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        q.wi[m] = S[(PS_W + m) * N + e];
+        q.di[m] = S[(PS_DW + m) * N + e];
+        q.rate[m] = S[(PS_RATE + m) * N + e];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) q.dl[r] = S[(PS_DL + r) * N + e];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) q.g[f] = S[(PS_G + f) * N + e];
+    // west / east neighbours; the seam reads the last sweep's copy
+    {
+        const bool seam = i == 0;
+        const int ni = seam ? nx - 1 : i - 1;
+        const size_t en = (size_t)(ni + j) * 64 + ni;
+        const int od = seam ? PS_DWS : PS_DW, ol = seam ? PS_DLS : PS_DL;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { q.dn[0][m] = S[(od + m) * N + en]; q.tj[0][m] = -S[(ol + m) * N + en]; }
+    }
+    {
+        const bool seam = i == nx - 1;
+        const int ni = seam ? 0 : i + 1;
+        const size_t en = (size_t)(ni + j) * 64 + ni;
+        const int od = seam ? PS_DWS : PS_DW, ol = seam ? PS_DLS : PS_DL;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { q.dn[1][m] = S[(od + m) * N + en]; q.tj[1][m] = S[(ol + m) * N + en]; }
+    }
+    // south / north; walls take the Zhu images of the cell itself
+    // (a wall face reads the cell's own entry here and is filled in by wall_face)
+#pragma unroll
+    for (int f = 2; f < 4; ++f) {
+        const int top = f == 3;
+        const bool wall = top ? j == ny - 1 : j == 0;
+        const size_t en = wall ? e : e + (top ? 64 : -64);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            q.dn[f][m] = S[(PS_DW + m) * N + en];
+            const double t = S[(PS_DL + 4 + m) * N + en];
+            q.tj[f][m] = top ? t : -t;
+        }
+    }
+    (void)gam;
+}
+
+// wall face f (2 bottom, 3 top) of a cell: the Zhu images of its own state
+// and increment stand for the neighbour
+__device__ __attribute__((always_inline)) inline void wall_face(const Pr2Ctx& D, MacroIn& q, int f) {
+    const int top = f == 3;
+    const double uw = D.uw[top], nyo = top ? 1.0 : -1.0;
+    double wj[4], a[4], fnew[4], fold[4];
+    zhu_map(q.wi, uw, 0.0, wj);
+    zhu_map(q.di, uw, 0.0, q.dn[f]);
+    for (int m = 0; m < 4; ++m) a[m] = wj[m] + q.dn[f][m];
+    euler_flux(D.gamma, a, 0.0, nyo, fnew);
+    euler_flux(D.gamma, wj, 0.0, nyo, fold);
+    for (int m = 0; m < 4; ++m) q.tj[f][m] = fnew[m] - fold[m];
+}
+
+// step s of a half sweep: relax the lane's cell of that diagonal from `cur`
+// while `nxt` receives the inputs of step s+1.  prev holds the lane's last
+// increment and its x and y flux differences.
+template <bool FWD>
+__device__ __attribute__((always_inline)) inline void macro_step(const Pr2Ctx& D, gdouble* __restrict__ S, size_t N,
+                                                                 int lane, int s, double dtp, double prev[12],
+                                                                 MacroIn& cur, MacroIn& nxt, int& ok) {
+    const int nx = D.nx, ny = D.ny, nd = nx + ny - 1;
+    constexpr int FX = FWD ? 0 : 1, FY = FWD ? 2 : 3;
+    constexpr double SG = FWD ? -1.0 : 1.0;    // sign of the upstream faces' normals
+    const int d = FWD ? s : nd - 1 - s;
+    const int j = d - lane;
+    const bool active = lane < nx && j >= 0 && j < ny;
+    {
+        // inputs of the next diagonal; lanes off the grid load a clamped cell
+        const int j1 = (FWD ? s + 1 : nd - 2 - s) - lane;
+        const int jc = j1 < 0 ? 0 : (j1 >= ny ? ny - 1 : j1), ic = lane < nx ? lane : nx - 1;
+        macro_load(D, S, N, ic, jc, nxt);
+    }
+    // upstream x neighbour: increment and x difference of lane i-1 (forward) or i+1 (back)
+    const int src = FWD ? (lane + 63) & 63 : (lane + 1) & 63;
+    double upd[4], upx[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { upd[m] = __shfl(prev[m], src, 64); upx[m] = __shfl(prev[4 + m], src, 64); }
+    if (active) {
+        if (j == 0) wall_face(D, cur, 2);
+        if (j == ny - 1) wall_face(D, cur, 3);
+        const bool seam = FWD ? lane == 0 : lane == nx - 1;
+        const bool wall = FWD ? j == 0 : j == ny - 1;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (!seam) { cur.dn[FX][m] = upd[m]; cur.tj[FX][m] = SG * upx[m]; }
+            if (!wall) { cur.dn[FY][m] = prev[m]; cur.tj[FY][m] = SG * prev[8 + m]; }
+        }
+        double sdt[4] = {0, 0, 0, 0}, sgn[4] = {0, 0, 0, 0}, sg = 0.0;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const double len = f < 2 ? D.dy : D.dx;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                // own difference for this face's normal: west/south are the negatives
+                const double t = cur.dl[(f < 2 ? 0 : 4) + m];
+                const double ti = (f == 0 || f == 2) ? -t : t;
+                sdt[m] += len * (ti + cur.tj[f][m]);
+                sgn[m] += len * cur.g[f] * cur.dn[f][m];
+            }
+            sg += len * cur.g[f];
+        }
+        const double area = D.dx * D.dy;
+        const double diag = 1.0 / dtp + 0.5 * sg / area;
+        bool good = isfinite(diag) && diag > 0.0;
+        double out[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double rhs = cur.rate[m] - 0.5 * sdt[m] / area + 0.5 * sgn[m] / area;
+            good = good && isfinite(rhs);
+            out[m] = rhs / diag;
+        }
+        ok &= good ? 1 : 0;
+        const size_t e = (size_t)d * 64 + lane;
+        if (good) {
+            double a[4], fa[4], fb[4], fx[4], fy[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = cur.wi[m] + out[m];
+            euler_flux(D.gamma, a, 1.0, 0.0, fa);
+            euler_flux(D.gamma, a, 0.0, 1.0, fb);
+            euler_flux(D.gamma, cur.wi, 1.0, 0.0, fx);
+            euler_flux(D.gamma, cur.wi, 0.0, 1.0, fy);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                prev[m] = out[m];
+                prev[4 + m] = fa[m] - fx[m];
+                prev[8 + m] = fb[m] - fy[m];
+                S[(PS_DW + m) * N + e] = out[m];
+                S[(PS_DL + m) * N + e] = prev[4 + m];
+                S[(PS_DL + 4 + m) * N + e] = prev[8 + m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                prev[m] = cur.di[m];
+                prev[4 + m] = cur.dl[m];
+                prev[8 + m] = cur.dl[4 + m];
+            }
+        }
+    }
+}
+
+// one half sweep (forward: diagonals 0..nd-1, else nd-1..0) of lane `lane`;
+// the two input buffers alternate roles (no copies between steps)
+template <bool FWD>
+__device__ __attribute__((always_inline)) inline void macro_half(const Pr2Ctx& D, gdouble* __restrict__ S, size_t N,
+                                                                 int lane, double dtp, int& ok) {
+    const int nx = D.nx, ny = D.ny, nd = nx + ny - 1;
+    double prev[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    MacroIn A = {}, B = {};
+    {
+        const int j = (FWD ? 0 : nd - 1) - lane;
+        const int jc = j < 0 ? 0 : (j >= ny ? ny - 1 : j), ic = lane < nx ? lane : nx - 1;
+        macro_load(D, S, N, ic, jc, A);
+    }
+    int s = 0;
+    for (; s + 1 < nd; s += 2) {
+        macro_step<FWD>(D, S, N, lane, s, dtp, prev, A, B, ok);
+        macro_step<FWD>(D, S, N, lane, s + 1, dtp, prev, B, A, ok);
+    }
+    if (s < nd) macro_step<FWD>(D, S, N, lane, s, dtp, prev, A, B, ok);
+}
+
+__global__ void __launch_bounds__(64) k_pr2_wave(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int lane = threadIdx.x;
+    const double dtp = D.scal[SC_DTP];
+    gdouble* S = (gdouble*)D.lines;
+    const size_t N = pr2_len(D);
+    const int sweeps = D.macro_sweeps;
+    const Pr2Ctx X{D.nx, D.ny, D.dx, D.dy, D.gas.gamma, {D.wall_ux[0], D.wall_ux[1]}};
+    int ok = 1;
+    for (int sweep = 0; sweep < sweeps; ++sweep) {
+        macro_half<true>(X, S, N, lane, dtp, ok);
+        __threadfence_block();
+        __syncthreads();
+        macro_half<false>(X, S, N, lane, dtp, ok);
+        __threadfence_block();
+        __syncthreads();
+        // the seam halo of the next sweep: this sweep's increments and differences
+        for (size_t e = lane; e < N; e += 64) {
+            for (int m = 0; m < 4; ++m) S[(PS_DWS + m) * N + e] = S[(PS_DW + m) * N + e];
+            for (int q = 0; q < 8; ++q) S[(PS_DLS + q) * N + e] = S[(PS_DL + q) * N + e];
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    ok = __syncthreads_and(ok);
+    if (lane == 0) {
+        D.scal[SC_PR2_OK] = ok ? 1.0 : 0.0;
+        if (!ok) atomicOr(D.status, (unsigned)ST_NONFINITE);
+    }
+}
+
+// the increments back to cell order and the predicted macro state
+__global__ void __launch_bounds__(256) k_pr2_finish(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.ncell) return;
+    const int nx = D.nx, i = c % nx, j = c / nx;
+    const size_t N = pr2_len(D), e = (size_t)(i + j) * 64 + i;
+    const double* S = D.lines;
+    bool good = true;
+    double t[4];
+    for (int m = 0; m < 4; ++m) {
+        const double d = S[(PS_DW + m) * N + e];
+        good = good && isfinite(d);
+        D.dW[4 * c + m] = d;
+        t[m] = D.W[4 * c + m] + d;
+        D.w_tilde[4 * c + m] = t[m];
+    }
+    good = good && physical_w(t);
+    // a non-finite relaxation is reported by k_pr2_wave alone (as by k_macro_pr2)
+    if (!good && D.scal[SC_PR2_OK] != 0.0) atomicOr(D.status, (unsigned)ST_MACRO_ALPHA);
+}
+
+// One lane per column, and the staging arrays (36 components of
+// (nx+ny-1)*64 entries) fit in the line buffer (nx*ny*22*Nvel doubles): with
+// Nvel >= 64 that holds for every grid of at least 5x5 cells, the smallest
+// the reconstruction takes.  Batched instances share nx and Nvel.
+inline bool pr2_wave_fits(const Dev& D) { return D.nx <= 64 && D.nvel >= 64 && D.nx >= 5 && D.ny >= 5; }
+
 // ===========================================================================
 // predicted equilibrium: per-cell right-hand side of the micro sweeps
 // ===========================================================================
@@ -184,18 +502,31 @@ struct CellVel {
     double u[KPT], v[KPT], w[KPT];
 };
 
-// Maxwellian plus Shakhov part at prim (rho,U,V,lambda) with heat flux q
-__device__ inline void maxwell_shakhov(const Gas& g, const double prim[4], const double q[2], double u, double v,
-                                       double* eh, double* eb) {
+// exponential factor exp(-lambda |c - U|^2) of the Maxwellian at prim
+__device__ inline double maxwell_exp(const double prim[4], double u, double v) {
+    double dvn = u - prim[1], dvt = v - prim[2], v2 = dvn * dvn + dvt * dvt;
+    return exp(-prim[3] * v2);
+}
+
+// Maxwellian plus Shakhov part at prim (rho,U,V,lambda) with heat flux q,
+// given ex = maxwell_exp(prim, u, v)
+__device__ inline void maxwell_shakhov_x(const Gas& g, const double prim[4], const double q[2], double u, double v,
+                                         double ex, double* eh, double* eb) {
     double pre = prim[0] * (prim[3] / UGKS_PI), bf = g.ck / (2.0 * prim[3]);
     double coeff = 0.8 * (1.0 - g.pr) * prim[3] * prim[3] / prim[0];
     double dvn = u - prim[1], dvt = v - prim[2], v2 = dvn * dvn + dvt * dvt;
-    double hk = pre * exp(-prim[3] * v2);
+    double hk = pre * ex;
     double bk = hk * bf;
     double qdot = dvn * q[0] + dvt * q[1];
     double shH = 2.0 * prim[3] * v2 - 4.0, shB = shH + 2.0 / g.ck;
     *eh = hk + coeff * qdot * shH * hk;
     *eb = bk + coeff * qdot * shB * bk;
+}
+
+// Maxwellian plus Shakhov part at prim (rho,U,V,lambda) with heat flux q
+__device__ inline void maxwell_shakhov(const Gas& g, const double prim[4], const double q[2], double u, double v,
+                                       double* eh, double* eb) {
+    maxwell_shakhov_x(g, prim, q, u, v, maxwell_exp(prim, u, v), eh, eb);
 }
 
 // Gram matrix of the weighted moment projection (upper triangle, 10 entries)
@@ -215,6 +546,10 @@ __device__ inline void gram_full(const double G[10], double M[16]) {
         for (int c = a; c < 4; ++c) { M[c * 4 + a] = G[n]; M[a * 4 + c] = G[n]; ++n; }
 }
 
+// Predicted equilibrium (src/mod_ImplicitMicro.f90:43-88).  KPT velocities per
+// thread; nothing per velocity stays live between the passes (each pass
+// reloads its inputs from the cache), so the 4096-velocity instance does not
+// spill.
 template <int KPT, int NT>
 __global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
@@ -222,13 +557,8 @@ __global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
     const int c = blockIdx.x, nvel = D.nvel;
     if (c >= D.ncell) return;
     const Gas& g = D.gas;
-    double u[KPT], v[KPT], w[KPT], h[KPT], b[KPT];
-#pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
-        u[r] = D.u[k]; v[r] = D.v[k]; w[r] = D.w[k];
-        h[r] = D.h[(size_t)c * nvel + k]; b[r] = D.b[(size_t)c * nvel + k];
-    }
+    const double* __restrict__ hc = D.h + (size_t)c * nvel;
+    const double* __restrict__ bc = D.b + (size_t)c * nvel;
     double W[4], Wt[4], po[4], pt[4];
     for (int m = 0; m < 4; ++m) { W[m] = D.W[4 * c + m]; Wt[m] = D.w_tilde[4 * c + m]; }
     primary(g.gamma, W, po);
@@ -237,21 +567,29 @@ __global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
     double q[2] = {0.0, 0.0};
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
-        double dvn = u[r] - po[1], dvt = v[r] - po[2], v2 = dvn * dvn + dvt * dvt;
-        q[0] += 0.5 * w[r] * dvn * (v2 * h[r] + b[r]);
-        q[1] += 0.5 * w[r] * dvt * (v2 * h[r] + b[r]);
+        const int k = threadIdx.x + NT * r;
+        const double u = D.u[k], v = D.v[k], w = D.w[k], h = hc[k], b = bc[k];
+        double dvn = u - po[1], dvt = v - po[2], v2 = dvn * dvn + dvt * dvt;
+        q[0] += 0.5 * w * dvn * (v2 * h + b);
+        q[1] += 0.5 * w * dvt * (v2 * h + b);
     }
     block_sum<2>(q, lds);
     const double tt = tau_of_prim(g, pt), to = tau_of_prim(g, po);
     // unprojected targets: moments and magnitudes
+    // (the exponentials of both Maxwellians are kept for the later passes)
     double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
+    double xt[KPT], xo[KPT];
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
+        const int k = threadIdx.x + NT * r;
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
         double eh, eb, lh, lb;
-        maxwell_shakhov(g, pt, q, u[r], v[r], &eh, &eb);
-        maxwell_shakhov(g, po, q, u[r], v[r], &lh, &lb);
-        mom_add(s8, w[r], u[r], v[r], eh, eb);
-        mom_add(s8 + 4, w[r], u[r], v[r], lh, lb);
+        xt[r] = maxwell_exp(pt, u, v);
+        xo[r] = maxwell_exp(po, u, v);
+        maxwell_shakhov_x(g, pt, q, u, v, xt[r], &eh, &eb);
+        maxwell_shakhov_x(g, po, q, u, v, xo[r], &lh, &lb);
+        mom_add(s8, w, u, v, eh, eb);
+        mom_add(s8 + 4, w, u, v, lh, lb);
         mx[0] = fmax(mx[0], fabs(eh)); mx[1] = fmax(mx[1], fabs(eb));
         mx[2] = fmax(mx[2], fabs(lh)); mx[3] = fmax(mx[3], fabs(lb));
     }
@@ -263,11 +601,13 @@ __global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
     for (int a = 0; a < 20; ++a) G[a] = 0.0;
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
+        const int k = threadIdx.x + NT * r;
+        const double u = D.u[k], v = D.v[k], w = D.w[k];
         double eh, eb, lh, lb;
-        maxwell_shakhov(g, pt, q, u[r], v[r], &eh, &eb);
-        maxwell_shakhov(g, po, q, u[r], v[r], &lh, &lb);
-        gram_add(G, w[r], u[r], v[r], fmax(fabs(eh), sc[0]), fmax(fabs(eb), sc[1]));
-        gram_add(G + 10, w[r], u[r], v[r], fmax(fabs(lh), sc[2]), fmax(fabs(lb), sc[3]));
+        maxwell_shakhov_x(g, pt, q, u, v, xt[r], &eh, &eb);
+        maxwell_shakhov_x(g, po, q, u, v, xo[r], &lh, &lb);
+        gram_add(G, w, u, v, fmax(fabs(eh), sc[0]), fmax(fabs(eb), sc[1]));
+        gram_add(G + 10, w, u, v, fmax(fabs(lh), sc[2]), fmax(fabs(lb), sc[3]));
     }
     block_sum<20>(G, lds);
     double Me[16], Ml[16], re[4], rl[4], le[4], ll[4];
@@ -275,29 +615,28 @@ __global__ void __launch_bounds__(NT) k_predict(const Dev* __restrict__ DS) {
     gram_full(G + 10, Ml);
     for (int m = 0; m < 4; ++m) { re[m] = Wt[m] - s8[m]; rl[m] = W[m] - s8[4 + m]; }
     unsigned st = 0;
-    if (!solve4_gj(Me, re, le) || !solve4_gj(Ml, rl, ll)) st = ST_PROJECT;
+    if (!solve4_gj_reg(Me, re, le) || !solve4_gj_reg(Ml, rl, ll)) st = ST_PROJECT;
     double chk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool fin = true;
-    const double dtp = D.scal[SC_DTP];
-    (void)dtp;
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
-        int k = threadIdx.x + NT * r;
+        const int k = threadIdx.x + NT * r;
+        const double u = D.u[k], v = D.v[k], w = D.w[k], h = hc[k], b = bc[k];
         double eh, eb, lh, lb;
-        maxwell_shakhov(g, pt, q, u[r], v[r], &eh, &eb);
-        maxwell_shakhov(g, po, q, u[r], v[r], &lh, &lb);
-        double kh = 0.5 * (u[r] * u[r] + v[r] * v[r]);
+        maxwell_shakhov_x(g, pt, q, u, v, xt[r], &eh, &eb);
+        maxwell_shakhov_x(g, po, q, u, v, xo[r], &lh, &lb);
+        double kh = 0.5 * (u * u + v * v);
         double weh = fmax(fabs(eh), sc[0]), web = fmax(fabs(eb), sc[1]);
         double wlh = fmax(fabs(lh), sc[2]), wlb = fmax(fabs(lb), sc[3]);
-        eh = eh + weh * (le[0] + u[r] * le[1] + v[r] * le[2] + kh * le[3]);
+        eh = eh + weh * (le[0] + u * le[1] + v * le[2] + kh * le[3]);
         eb = eb + 0.5 * web * le[3];
-        lh = lh + wlh * (ll[0] + u[r] * ll[1] + v[r] * ll[2] + kh * ll[3]);
+        lh = lh + wlh * (ll[0] + u * ll[1] + v * ll[2] + kh * ll[3]);
         lb = lb + 0.5 * wlb * ll[3];
-        mom_add(chk, w[r], u[r], v[r], eh, eb);
-        mom_add(chk + 4, w[r], u[r], v[r], lh, lb);
+        mom_add(chk, w, u, v, eh, eb);
+        mom_add(chk + 4, w, u, v, lh, lb);
         size_t o = (size_t)c * nvel + k;
-        double rh = D.rh[o] + (eh - h[r]) / tt - (lh - h[r]) / to;
-        double rb = D.rb[o] + (eb - b[r]) / tt - (lb - b[r]) / to;
+        double rh = D.rh[o] + (eh - h) / tt - (lh - h) / to;
+        double rb = D.rb[o] + (eb - b) / tt - (lb - b) / to;
         fin = fin && isfinite(rh) && isfinite(rb);
         D.rh[o] = rh;
         D.rb[o] = rb;
@@ -681,7 +1020,16 @@ extern "C" hipError_t ugks_launch_implicit(const LaunchSet* L, int rescale, hipS
     const Dev* DS = L->dev;
     int ng = (D.ncell + (D.ny + 1) * D.nx + 255) / 256;
     UGKS_TIMED(P, K_MACRO_GAMMA, s, hipLaunchKernelGGL(k_macro_gamma, dim3(ng, 1, n), dim3(256), 0, s, DS));
-    UGKS_TIMED(P, K_MACRO_PR2, s, hipLaunchKernelGGL(k_macro_pr2, dim3(1, 1, n), dim3(256), 0, s, DS));
+    if (pr2_wave_fits(D)) {
+        const int nb = (D.ncell + 255) / 256;
+        UGKS_TIMED(P, K_MACRO_PR2, s, {
+            hipLaunchKernelGGL(k_pr2_stage, dim3(nb, 1, n), dim3(256), 0, s, DS);
+            hipLaunchKernelGGL(k_pr2_wave, dim3(1, 1, n), dim3(64), 0, s, DS);
+            hipLaunchKernelGGL(k_pr2_finish, dim3(nb, 1, n), dim3(256), 0, s, DS);
+        });
+    } else {
+        UGKS_TIMED(P, K_MACRO_PR2, s, hipLaunchKernelGGL(k_macro_pr2, dim3(1, 1, n), dim3(256), 0, s, DS));
+    }
     switch (D.nvel) {
     case 256:
         UGKS_TIMED(P, K_PREDICT, s, hipLaunchKernelGGL((k_predict<4, 64>), dim3(D.ncell, 1, n), dim3(64), 0, s, DS));

@@ -27,6 +27,7 @@ enum ScalarSlot {
     SC_MB4 = 96,        // 4 totals sum A*W of the proposal before the rescale
     SC_LEDGER_NEXT = 100,  // 4 ledger if the current proposal is accepted
     SC_MSTAT = 104,     // 11 mass statistics of the accepted state: totals(5), metrics(5), ok
+    SC_PR2_OK = 116,    // 1 when every PR(2) relaxation of the update was finite
     SC_COUNT = 120
 };
 
@@ -68,7 +69,8 @@ struct Dev {
     double *Wn, *hn, *bn;
     // reconstruction outputs (upwind face data, [face][node][6][nvel])
     double *xf, *yf, *wb, *wt;
-    // first-stage lines of h and b: lines[((cell*LINE_ROWS + row)*nvel + k)*2 + field]
+    // first-stage lines of h and b: lines[((cell*LINE_ROWS + row)*nvel + k)*2 + field];
+    // during the implicit update it holds the PR(2) staging arrays instead
     double* lines;
     // macro reconstruction: ws[cell][m][side][node][2] (dn, dt per cell width), wv[cell][m][9]
     double *ws, *wv;
