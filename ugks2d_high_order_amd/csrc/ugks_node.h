@@ -91,7 +91,15 @@ __device__ inline void store_side(double* rec, size_t nv, int off, double theta,
 // held in registers across a workgroup reduction.  Per-velocity values shared
 // by several passes live in LDS slots indexed by velocity; a slot is only ever
 // touched by the thread that owns the velocity, so the slots need no barriers.
-#define UGKS_VLOOP(k, NT, n) _Pragma("unroll 1") for (int k = threadIdx.x; k < (n); k += (NT))
+// UGKS_VLOOP_UNROLL velocities of a thread are in flight together (their
+// loads issued back to back); a translation unit may set it before including
+// this header.
+#ifndef UGKS_VLOOP_UNROLL
+#define UGKS_VLOOP_UNROLL 1
+#endif
+#define UGKS_PRAGMA_(x) _Pragma(#x)
+#define UGKS_PRAGMA(x) UGKS_PRAGMA_(x)
+#define UGKS_VLOOP(k, NT, n) UGKS_PRAGMA(unroll UGKS_VLOOP_UNROLL) for (int k = threadIdx.x; k < (n); k += (NT))
 
 // q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q: the moments of q and of
 // dq/dtheta_a (Jacobian columns), red[0..3] moments, red[4+4a..] column a.

@@ -10,6 +10,7 @@
 //                                                   src/mod_S2NativeAdapter.f90:735-759
 //   UpdateImplicitResidualDiagnostics               src/mod_ImplicitDiagnostics.f90:223-322
 //   TimeStep                                         src/mod_Solve.f90:373-421
+#define UGKS_VLOOP_UNROLL 2   // two velocities of a thread in flight per pass
 #include "ugks_node.h"
 #include "ugks_prof.h"
 
@@ -174,9 +175,10 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
 // b_t} of nvel values.  The flux times the Gauss weight is added into oh/ob
 // (written on the first node), the macro flux into fw_acc.
 template <int NT>
-__device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const double* rec, const double* sl_p,
-                                     const double* sr_p, double ln, double lt, double gwa, bool first, double* oh,
-                                     double* ob, double fw_acc[4], double* qh, double* lds, double* tape) {
+__device__ inline unsigned ugks_node(const Dev& D, int axis, double dt, const double* __restrict__ rec,
+                                     const double* sl_p, const double* sr_p, double ln, double lt, double gwa,
+                                     bool first, double* __restrict__ oh, double* __restrict__ ob, double fw_acc[4],
+                                     double* qh, double* lds, double* tape) {
     const int ck = D.gas.ck, nvel = D.nvel;
     const size_t nv = (size_t)nvel;
     double w0[4] = {0, 0, 0, 0};
