@@ -349,6 +349,7 @@ class UGKS:
     TUNE_VEL_WG = 1
     TUNE_EXPLICIT_WG = 2
     TUNE_SWEEP_TILE = 3
+    TUNE_FACE_WG = 4
 
     def set_tuning(self, key: int, value: int):
         _check(load_library().ugks_set_tuning(self.handle, int(key), int(value)))

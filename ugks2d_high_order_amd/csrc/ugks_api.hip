@@ -260,6 +260,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
     // 256-thread velocity workgroup for the largest velocity grid, and a sweep
     // tile that gives about one workgroup slot per cell of the longest diagonal
     D.wg_vel = nvel >= 4096 ? 256 : 128;
+    D.wg_face = 128;
     D.tile = 256 / std::max(8, std::min(32, p->nx));
     D.tile = D.tile < MIN_TILE ? MIN_TILE : (D.tile > 32 ? 32 : D.tile);
     D.ntiles = nvel / D.tile;
@@ -946,7 +947,8 @@ extern "C" int ugks_profile_read(void* handle, int idx, const char** name, doubl
 }
 
 // Launch-shape tuning (no counterpart in the reference).  key 1: workgroup
-// size of the velocity-parallel operator kernels (64, 128 or 256).
+// size of the velocity-parallel cell kernels (64, 128 or 256); key 4: that of
+// the face and wall flux kernels.
 extern "C" int ugks_set_tuning(void* handle, int key, int value) {
     Instance* I = (Instance*)handle;
     if (!I) return fail(E_ARG, "null handle");
@@ -954,6 +956,10 @@ extern "C" int ugks_set_tuning(void* handle, int key, int value) {
     case 1:
         if (value != 64 && value != 128 && value != 256) return fail(E_ARG, "velocity workgroup must be 64, 128 or 256");
         I->D.wg_vel = value;
+        return upload(I);
+    case 4:
+        if (value != 64 && value != 128 && value != 256) return fail(E_ARG, "face workgroup must be 64, 128 or 256");
+        I->D.wg_face = value;
         return upload(I);
     case 3:
         if (value != 8 && value != 16 && value != 32) return fail(E_ARG, "sweep velocity tile must be 8, 16 or 32");
@@ -979,6 +985,7 @@ extern "C" int ugks_get_tuning(void* handle, int key, int* value) {
     case 1: *value = I->D.wg_vel; return E_OK;
     case 2: *value = I->lo_wg; return E_OK;
     case 3: *value = I->D.tile; return E_OK;
+    case 4: *value = I->D.wg_face; return E_OK;
     default: return fail(E_ARG, "unknown tuning key");
     }
 }

@@ -1053,33 +1053,50 @@ extern "C" hipError_t ugks_launch_observables(const LaunchSet* L, double* out, h
     return hipGetLastError();
 }
 
-// face, wall and cell kernels with NT-thread workgroups
+// face and wall flux kernels with NT-thread workgroups
 template <int NT>
-static hipError_t launch_closure_kernels(const LaunchSet& L, hipStream_t s, Prof* P) {
-    static size_t lds_face = 0, lds_wall = 0, lds_cell = 0;
+static hipError_t launch_face_kernels(const LaunchSet& L, hipStream_t s, Prof* P) {
+    static size_t lds_face = 0, lds_wall = 0;
     const Dev& D = *L.size;
-    const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
+    const size_t q1 = sizeof(double) * D.nvel;
     hipError_t e;
     if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
     if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
-    if ((e = allow_lds(k_cell<NT>, q2, &lds_cell)) != hipSuccess) return e;
     UGKS_TIMED(P, K_FACE_FLUX, s,
                hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1), 1, L.n), dim3(NT), q1, s, L.dev));
     UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx, 1, L.n), dim3(NT), q1, s, L.dev));
+    return hipGetLastError();
+}
+
+static hipError_t launch_faces(const LaunchSet& L, hipStream_t s, Prof* P) {
+    switch (L.size->wg_face) {
+    case 64: return launch_face_kernels<64>(L, s, P);
+    case 128: return launch_face_kernels<128>(L, s, P);
+    case 256: return launch_face_kernels<256>(L, s, P);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// cell kernel with NT-thread workgroups
+template <int NT>
+static hipError_t launch_cell_kernel(const LaunchSet& L, hipStream_t s, Prof* P) {
+    static size_t lds_cell = 0;
+    const Dev& D = *L.size;
+    const size_t q2 = 2 * sizeof(double) * D.nvel;
+    hipError_t e;
+    if ((e = allow_lds(k_cell<NT>, q2, &lds_cell)) != hipSuccess) return e;
     UGKS_TIMED(P, K_CELL, s, hipLaunchKernelGGL(k_cell<NT>, dim3(D.ncell, 1, L.n), dim3(NT), q2, s, L.dev));
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 template <int NT>
 static hipError_t launch_cn(const LaunchSet& L, bool build_old, hipStream_t s, Prof* P) {
-    static size_t lds_t = 0, lds_c = 0, lds_face = 0, lds_wall = 0;
+    static size_t lds_t = 0, lds_c = 0;
     const Dev& D = *L.size;
     const size_t q1 = sizeof(double) * D.nvel, q2 = 2 * q1;
     hipError_t e;
     if ((e = allow_lds(k_cn_targets<NT>, q2, &lds_t)) != hipSuccess) return e;
     if ((e = allow_lds(k_cn_cell<NT>, q2, &lds_c)) != hipSuccess) return e;
-    if ((e = allow_lds(k_face_flux<NT>, q1, &lds_face)) != hipSuccess) return e;
-    if ((e = allow_lds(k_wall_flux<NT>, q1, &lds_wall)) != hipSuccess) return e;
     const int n = L.n;
     UGKS_TIMED(P, K_TIMESTEP, s, hipLaunchKernelGGL(k_timestep, dim3(1, 1, n), dim3(256), 0, s, L.dev));
     const dim3 gw((4 * D.nx + 255) / 256, D.ny, n);
@@ -1089,9 +1106,7 @@ static hipError_t launch_cn(const LaunchSet& L, bool build_old, hipStream_t s, P
     UGKS_TIMED(P, K_RECON_LINES, s,
                hipLaunchKernelGGL(k_recon_lines, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
     UGKS_TIMED(P, K_RECON, s, hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L.dev));
-    UGKS_TIMED(P, K_FACE_FLUX, s,
-               hipLaunchKernelGGL(k_face_flux<NT>, dim3(D.ncell + D.nx * (D.ny - 1), 1, n), dim3(NT), q1, s, L.dev));
-    UGKS_TIMED(P, K_WALL_FLUX, s, hipLaunchKernelGGL(k_wall_flux<NT>, dim3(2 * D.nx, 1, n), dim3(NT), q1, s, L.dev));
+    if ((e = launch_faces(L, s, P)) != hipSuccess) return e;
     UGKS_TIMED(P, K_CN_STREAM, s,
                hipLaunchKernelGGL(k_cn_stream, dim3((D.nvel + 255) / 256, D.ncell, n), dim3(256), 0, s, L.dev));
     UGKS_TIMED(P, K_WRECON, s, hipLaunchKernelGGL(k_wrecon, gw, dim3(256), 0, s, L.dev, 1));
@@ -1136,11 +1151,12 @@ extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Pr
                hipLaunchKernelGGL(k_recon_lines, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L->dev));
     UGKS_TIMED(P, K_RECON, s,
                hipLaunchKernelGGL(k_recon, dim3(2 * D.nvel / 128, D.ncell, n), dim3(128), 0, s, L->dev));
-    hipError_t e;
+    hipError_t e = launch_faces(*L, s, P);
+    if (e != hipSuccess) return e;
     switch (D.wg_vel) {
-    case 64: e = launch_closure_kernels<64>(*L, s, P); break;
-    case 128: e = launch_closure_kernels<128>(*L, s, P); break;
-    case 256: e = launch_closure_kernels<256>(*L, s, P); break;
+    case 64: e = launch_cell_kernel<64>(*L, s, P); break;
+    case 128: e = launch_cell_kernel<128>(*L, s, P); break;
+    case 256: e = launch_cell_kernel<256>(*L, s, P); break;
     default: e = hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;

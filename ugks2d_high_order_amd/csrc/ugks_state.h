@@ -60,7 +60,8 @@ struct Dev {
     double wall_ux[2], wall_tn[2];
     int macro_sweeps;
     int tile;             // velocities per sweep workgroup
-    int wg_vel;           // workgroup size of the velocity-parallel operator kernels (64, 128 or 256)
+    int wg_vel;           // workgroup size of the velocity-parallel cell kernels (64, 128 or 256)
+    int wg_face;          // workgroup size of the face and wall flux kernels (64, 128 or 256)
     int ntiles;
     // velocity table
     const double *u, *v, *w;
