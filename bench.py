@@ -66,14 +66,14 @@ EXPLICIT_BYTES_PER_CV = 4 * 8
 # (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
 KERNEL_COUNTS = {
     # kernel: (flop per unit, unit, HBM bytes per launch); 64x512 cells x 64x64 velocities
-    "k_recon": (7706.4, "cv", 1.2513e11),
-    "k_face_flux": (1916.1, "fv", 8.5755e10),
-    "k_cell": (3250.3, "cv", 1.2915e10),
-    "k_recon_lines": (1760.5, "cv", 3.4334e10),
-    "k_predict": (579.0, "cv", 3.5883e10),
+    "k_recon": (7672.9, "cv", 1.2513e+11),
+    "k_face_flux": (1755.2, "fv", 8.6001e+10),
+    "k_cell": (3250.3, "cv", 1.2740e+10),
+    "k_recon_lines": (1760.5, "cv", 3.4335e+10),
+    "k_predict": (605.0, "cv", 7.4426e+09),
 }
 # measured HBM bytes of one Explicit step (both face launches, walls, cells)
-EXPLICIT_TRAFFIC_PER_STEP = 1.6403e10 + 1.9873e10 + 1.2973e10
+EXPLICIT_TRAFFIC_PER_STEP = 4.9064e+10
 
 MI355X_HBM_GBPS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
