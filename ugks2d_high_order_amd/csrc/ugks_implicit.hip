@@ -667,7 +667,7 @@ struct MicroCoef {
 // forward sweep: delta = (R + upwind new neighbours)/D; neighbours across the
 // x seam and on the old side are zero (reset), the wall closure of the
 // forward sweep is frozen from a zero increment
-__global__ void __launch_bounds__(256) k_sweep_fwd(const Dev* __restrict__ DS) {
+__global__ void __launch_bounds__(1024) k_sweep_fwd(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel;
@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(256) k_sweep_fwd(const Dev* __restrict__ DS) {
 
 // backward sweep with the apply step: f' = f + delta (alpha = 1), per-tile
 // moment partials of f' for the macro recovery
-__global__ void __launch_bounds__(256) k_sweep_bwd(const Dev* __restrict__ DS) {
+__global__ void __launch_bounds__(1024) k_sweep_bwd(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     UGKS_DYN_SHARED(double, sm);   // [2][2][nx][T] + [2][nx]
     const int T = D.tile, nx = D.nx, ny = D.ny, nvel = D.nvel, ck = D.gas.ck;
@@ -1044,8 +1044,11 @@ extern "C" hipError_t ugks_launch_implicit(const LaunchSet* L, int rescale, hipS
     }
     size_t shf = sizeof(double) * 4 * D.nx * D.tile;
     size_t shb = shf + sizeof(double) * 2 * D.nx;
-    UGKS_TIMED(P, K_SWEEP_FWD, s, hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles, 1, n), dim3(256), shf, s, DS));
-    UGKS_TIMED(P, K_SWEEP_BWD, s, hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles, 1, n), dim3(256), shb, s, DS));
+    // one slot of tile lanes per cell of the longest diagonal (at most 1024
+    // threads), so that a diagonal is one pass between two barriers
+    const int swg = std::min(1024, std::max(256, (D.nx * D.tile + 63) / 64 * 64));
+    UGKS_TIMED(P, K_SWEEP_FWD, s, hipLaunchKernelGGL(k_sweep_fwd, dim3(D.ntiles, 1, n), dim3(swg), shf, s, DS));
+    UGKS_TIMED(P, K_SWEEP_BWD, s, hipLaunchKernelGGL(k_sweep_bwd, dim3(D.ntiles, 1, n), dim3(swg), shb, s, DS));
     UGKS_TIMED(P, K_RECOVER, s, hipLaunchKernelGGL(k_recover, dim3((D.ncell + 255) / 256, 1, n), dim3(256), 0, s, DS));
     UGKS_TIMED(P, K_MASS, s, hipLaunchKernelGGL(k_mass, dim3(1, 1, n), dim3(256), 0, s, DS, rescale));
     UGKS_TIMED(P, K_RESCALE, s, hipLaunchKernelGGL(k_rescale, dim3(1024, 1, n), dim3(256), 0, s, DS));
