@@ -283,7 +283,7 @@ extern "C" int ugks_create(const ugks_params_t* p, const double* u, const double
         (rc = dmalloc(I, &D.wb, nx * REC_ROWS * nvel)) || (rc = dmalloc(I, &D.wt, nx * REC_ROWS * nvel)) ||
         (rc = dmalloc(I, &D.lines, nc * LINE_ROWS * 2 * nvel)) ||
         (rc = dmalloc(I, &D.ws, nc * 4 * 24)) || (rc = dmalloc(I, &D.wv, nc * 4 * 9)) ||
-        (rc = dmalloc(I, &D.cellpart, nc * 16)) || (rc = dmalloc(I, &D.gx, nc)) ||
+        (rc = dmalloc(I, &D.cellpart, nc * 16)) || (rc = dmalloc(I, &D.normpart, NORM_BLOCKS * 16)) || (rc = dmalloc(I, &D.gx, nc)) ||
         (rc = dmalloc(I, &D.gy, (ny + 1) * nx)) || (rc = dmalloc(I, &D.tau_t, nc)) ||
         (rc = dmalloc(I, &D.wallpart, (size_t)(nvel / MIN_TILE) * 2 * nx)) ||
         (rc = dmalloc(I, &D.mompart, (size_t)(nvel / MIN_TILE) * nc * 4)) || (rc = dmalloc(I, &D.scal, SC_COUNT)) ||

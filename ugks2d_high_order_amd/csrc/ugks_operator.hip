@@ -897,15 +897,29 @@ __global__ void __launch_bounds__(256) k_cn_finish(const Dev* __restrict__ DS) {
 // ===========================================================================
 // residual diagnostics (one workgroup): sums of the per-cell partials
 // ===========================================================================
+// first stage: NORM_BLOCKS workgroups, each summing a contiguous range of
+// cells' partials (fixed order, so the norms are reproducible)
+__global__ void __launch_bounds__(256) k_norms_part(const Dev* __restrict__ DS) {
+    const Dev& D = DS[blockIdx.z];
+    __shared__ double lds[16 * 14];
+    const int per = (D.ncell + NORM_BLOCKS - 1) / NORM_BLOCKS;
+    const int c0 = blockIdx.x * per, c1 = c0 + per < D.ncell ? c0 + per : D.ncell;
+    double s[14];
+    for (int q = 0; q < 14; ++q) s[q] = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+        for (int q = 0; q < 14; ++q) s[q] += D.cellpart[(size_t)c * 16 + q];
+    block_sum<14>(s, lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 14; ++q) D.normpart[blockIdx.x * 16 + q] = s[q];
+}
+
 __global__ void __launch_bounds__(256) k_norms(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
     const double residual_tol = D.residual_tol;
-    __shared__ double lds[16 * 14];
     double s[14];
     for (int q = 0; q < 14; ++q) s[q] = 0.0;
-    for (int c = threadIdx.x; c < D.ncell; c += blockDim.x)
-        for (int q = 0; q < 14; ++q) s[q] += D.cellpart[(size_t)c * 16 + q];
-    block_sum<14>(s, lds);
+    for (int b = 0; b < NORM_BLOCKS; ++b)
+        for (int q = 0; q < 14; ++q) s[q] += D.normpart[b * 16 + q];
     if (threadIdx.x != 0) return;
     const double SMV = 1.17549435082228750797e-38;
     double* o = D.scal + SC_NORMS;
@@ -1160,6 +1174,9 @@ extern "C" hipError_t ugks_launch_operator(const LaunchSet* L, hipStream_t s, Pr
     default: e = hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
-    UGKS_TIMED(P, K_NORMS, s, hipLaunchKernelGGL(k_norms, dim3(1, 1, n), dim3(256), 0, s, L->dev));
+    UGKS_TIMED(P, K_NORMS, s, {
+        hipLaunchKernelGGL(k_norms_part, dim3(NORM_BLOCKS, 1, n), dim3(256), 0, s, L->dev);
+        hipLaunchKernelGGL(k_norms, dim3(1, 1, n), dim3(64), 0, s, L->dev);
+    });
     return hipGetLastError();
 }

@@ -46,6 +46,8 @@ enum { FT_LEN = 40, WT_LEN = 44, TT_LEN = 32, CT_LEN = 16, FJ_W = 12, WJ_W = 24 
 // 6a..6a+5) the limited h value, h normal and tangential derivatives (per
 // unit length), b value, b derivatives.
 enum { REC_ROWS = 18 };
+// workgroups of the first stage of the residual-norm reduction
+enum { NORM_BLOCKS = 64 };
 
 // One instance's descriptor.  Kernels receive a device array of them and
 // work on DS[blockIdx.z]: a plain launch has one instance, a batched launch
@@ -81,6 +83,7 @@ struct Dev {
     double *rh, *rb;
     double *rate_w;       // [ncell][4]
     double *cellpart;     // [ncell][16] per-cell residual partial sums
+    double *normpart;     // [NORM_BLOCKS][16] per-block sums of cellpart
     // implicit
     double *dW, *dWs;     // [ncell][4] macro increment and its last-exchange snapshot
     double *gx, *gy;      // spectral radii: east face of cell, south face of row j
