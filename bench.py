@@ -72,6 +72,8 @@ KERNEL_COUNTS = {
     "k_recon_lines": (1760.5, "cv", 3.4335e+10),
     "k_predict": (605.0, "cv", 7.4426e+09),
 }
+# measured FP64 operations of one Explicit step (both face launches, walls, cells)
+EXPLICIT_FLOP_PER_STEP = 2.1244e11
 # measured HBM bytes of one Explicit step (both face launches, walls, cells)
 EXPLICIT_TRAFFIC_PER_STEP = 4.9064e+10
 
@@ -377,6 +379,10 @@ def main():
             "roofline": {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
                          "frac": ach / MI355X_HBM_GBPS, "traffic": EXPLICIT_TRAFFIC_PER_STEP,
                          "basis": "whole step, 32 algorithmic bytes per cell-velocity (read h,b; write h,b)"},
+            # the step is FP64-bound: its counted FP64 operations against the vector peak
+            "fp64": {"achieved": EXPLICIT_FLOP_PER_STEP / (el / a.steps) / 1e12, "peak": MI355X_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": EXPLICIT_FLOP_PER_STEP / (el / a.steps) / 1e12 / MI355X_FP64_TFLOPS,
+                     "flop_per_step": EXPLICIT_FLOP_PER_STEP},
             "kernel_avg_ms": _kernel_table(kt),
             "cpu_baseline": None if a.no_cpu else cpu_baseline_explicit(case, threads)}
     if "batched" in legs:
