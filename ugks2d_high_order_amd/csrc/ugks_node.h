@@ -103,18 +103,26 @@ __device__ inline void store_side(double* rec, size_t nv, int off, double theta,
 
 // q(theta) = rho/(pi T) exp(-c^2/T), qb = ck/2 T q: the moments of q and of
 // dq/dtheta_a (Jacobian columns), red[0..3] moments, red[4+4a..] column a.
+// With qs, q of each owned velocity also goes to its LDS slot and *pos
+// records whether all of them are positive and finite.
 template <int NT>
-__device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds) {
+__device__ inline void q_pass(int ck, const Dev& D, const double th[4], double red[20], double* lds,
+                              double* qs = nullptr, int* pos = nullptr) {
 #pragma unroll
     for (int a = 0; a < 20; ++a) red[a] = 0.0;
     const double r0 = th[0], T = uni(th[3]), t1 = uni(th[1]), t2 = uni(th[2]);
     const double pre = uni(r0 / (UGKS_PI * T));
     const double i0 = uni(1.0 / r0), iT = uni(1.0 / T);
     const int nvel = D.nvel;
+    int ok = 1;
     UGKS_VLOOP(k, NT, nvel) {
         const double u = D.u[k], v = D.v[k], w = D.w[k];
         double du = u - t1, dv = v - t2, c2 = du * du + dv * dv;
         double q = pre * exp(-c2 * iT);
+        if (qs) {
+            qs[k] = q;
+            ok &= (q > 0.0 && isfinite(q)) ? 1 : 0;
+        }
         double qb = 0.5 * (double)ck * T * q;
         double l1 = 2.0 * du * iT, l2 = 2.0 * dv * iT, l3 = -iT + c2 * (iT * iT);
         double lb3 = l3 + iT;
@@ -124,6 +132,7 @@ __device__ inline void q_pass(int ck, const Dev& D, const double th[4], double r
         mom_add(red + 12, w, u, v, q * l2, qb * l2);
         mom_add(red + 16, w, u, v, q * l3, qb * lb3);
     }
+    if (pos) *pos = ok;
     block_sum<20>(red, lds);
 }
 
@@ -155,8 +164,11 @@ __device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], do
     th[2] = w[2] / w[0];
     th[3] = 4.0 * (w[3] / w[0] - 0.5 * (th[1] * th[1] + th[2] * th[2])) / (double)(ck + 2);
     if (!(th[3] > 0.0) || !isfinite(th[3])) return false;
+    // with STORE every q pass also leaves q in the slots; the last pass run
+    // is the one of the returned th, so no separate pass is needed
     double red[20], res[4];
-    q_pass<NT>(ck, D, th, red, lds);
+    int pos = 1;
+    q_pass<NT>(ck, D, th, red, lds, STORE ? qh : nullptr, &pos);
 #pragma unroll
     for (int a = 0; a < 16; ++a) J[a] = red[4 + a];
 #pragma unroll
@@ -176,7 +188,7 @@ __device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], do
             for (int a = 0; a < 4; ++a) nt[a] = th[a] + alpha * step[a];
             if (nt[0] > 0.0 && nt[3] > 0.0 && isfinite(nt[0]) && isfinite(nt[1]) && isfinite(nt[2]) &&
                 isfinite(nt[3])) {
-                q_pass<NT>(ck, D, nt, red, lds);
+                q_pass<NT>(ck, D, nt, red, lds, STORE ? qh : nullptr, &pos);
                 bool fin = true;
 #pragma unroll
                 for (int a = 0; a < 20; ++a) fin = fin && isfinite(red[a]);
@@ -204,13 +216,6 @@ __device__ inline bool closure_solve(int ck, const Dev& D, const double w[4], do
     }
     if (!(nrm <= UGKS_NEWTON_TOL)) return false;
     if (!STORE) return true;
-    const double pre = th[0] / (UGKS_PI * th[3]), iT = 1.0 / th[3];
-    int pos = 1;
-    UGKS_VLOOP(k, NT, D.nvel) {
-        double q = q_of(th, pre, iT, D.u[k], D.v[k]);
-        qh[k] = q;
-        pos &= (q > 0.0 && isfinite(q)) ? 1 : 0;
-    }
     return __syncthreads_and(pos) != 0;
 }
 
