@@ -34,7 +34,13 @@ Two more legs go into the same JSON line:
   Couette instances at 8x64 cells x 32x32 velocities recover hidden
   (Kn, wall Mach) from their velocity and temperature profiles after 20
   S2CN2 steps by 50 quasi-Newton iterations on the batched forward and the
-  batched adjoint; recovery errors and the time per iteration.
+  batched adjoint; recovery errors and the time per iteration;
+- "restart" (only with --legs restart): a complete restart of the
+  64x512 x 64x64 state (Mesh2d.dat, flow2d.dat, the 2.1 GB vdf2d.dat with its
+  Fortran subrecords, grid2d.grd, closed_mass_rank*.bin for 1x8 ranks,
+  residual_reference.in) written from the device and loaded into a new
+  instance: seconds each way, host GB/s, and whether the loaded state equals
+  the saved one bit for bit.
 """
 from __future__ import annotations
 
@@ -327,6 +333,37 @@ def leg_calibration(iterations=50):
     return res
 
 
+def leg_restart(U, case):
+    import tempfile
+    import numpy as np
+    from ugks2d_high_order_amd import restart
+    g = U.UGKS(case)
+    g.cold_start()
+    g.implicit_step()
+    g.evolution(outputs=False)
+    W0, h0, b0 = g.state()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        t0 = time.perf_counter()
+        restart.save_restart(g, d, step=1, np_y=8)
+        t1 = time.perf_counter()
+        size = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        g2 = U.UGKS(case)
+        t2 = time.perf_counter()
+        step, epoch = restart.load_restart(g2, d)
+        t3 = time.perf_counter()
+        W, h, b = g2.state()
+        exact = bool(np.array_equal(W, W0) and np.array_equal(h, h0) and np.array_equal(b, b0))
+        m0a, la = g.mass_target()
+        m0b, lb = g2.mass_target()
+        exact = exact and m0a == m0b and bool(np.array_equal(la, lb)) and g.references() == g2.references()
+        g2.close()
+    g.close()
+    return {"workload": "complete restart of the 64x512 x 64x64 HighOrderImplicit state after one update "
+                        "(closed-mass companions for NP_X x NP_Y = 1 x 8)",
+            "bytes": size, "save_s": t1 - t0, "load_s": t3 - t2, "save_GBps": size / (t1 - t0) / 1e9,
+            "load_GBps": size / (t3 - t2) / 1e9, "loaded_state_bitwise_equal": exact, "step": step}
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -391,6 +428,8 @@ def main():
         out["adjoint"] = leg_adjoint()
     if "calibration" in legs:
         out["calibration"] = leg_calibration()
+    if "restart" in legs:
+        out["restart"] = leg_restart(U, case)
     print(json.dumps(out), flush=True)
 
 

@@ -217,11 +217,12 @@ def read_flow2d(path: str, nx: int, ny: int):
 
 def write_vdf2d(path: str, h, b, nx: int, ny: int):
     """vdf2d.dat of the structured-quadrature branch; h and b are streamed
-    one velocity plane at a time (no full transposed copy)."""
+    64 velocity planes at a time (no full transposed copy)."""
     nv = h.shape[-1]
     h = np.asarray(h).reshape(ny * nx, nv)
     b = np.asarray(b).reshape(ny * nx, nv)
-    planes = (np.ascontiguousarray(a[:, k], dtype="<f8") for a in (h, b) for k in range(nv))
+    # velocity planes in file order, transposed 64 at a time
+    planes = (np.ascontiguousarray(a[:, k0:k0 + 64].T, dtype="<f8") for a in (h, b) for k0 in range(0, nv, 64))
     with open(path, "wb") as f:
         write_record(f, [np.array([1], dtype="<i4")])
         write_record(f, [np.array([nx, ny, 2 * nv], dtype="<i4")])
