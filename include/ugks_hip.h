@@ -202,6 +202,15 @@ int ugks_observables(void* handle, double* fields10);
  * (HIP device pointers, same layouts). */
 int ugks_set_state_device(void* handle, const double* W, const double* h, const double* b);
 int ugks_get_state_device(void* handle, double* W, double* h, double* b);
+/* The state with the distributions as velocity planes: planes[2][Nvel][cell]
+ * holds h then b, plane k in cell order c = (j-1)*nx + (i-1), which is the
+ * third record of vdf2d.dat (Output_restart_only / InitFlow_Restart,
+ * src/mod_Output.f90:38-56, src/mod_InitializeFlow.f90 restart branch).  The
+ * transpose is done on the device through a 64-plane stage, so a restart is
+ * written and read with no host-side transpose.  W is the [cell][4] array of
+ * ugks_get_state; set resets the operator as ugks_set_state does. */
+int ugks_get_state_planes(void* handle, double* W, double* planes);
+int ugks_set_state_planes(void* handle, const double* W, const double* planes);
 /* Synchronize the instance's stream. */
 int ugks_sync(void* handle);
 

@@ -124,6 +124,12 @@ def test_subrecords(total):
         raw = f.getvalue()
         f.seek(0)
         assert R.read_record(f) == data and f.read() == b""
+        if total % 8 == 0:
+            f.seek(0)
+            assert R.read_real_record(f, total // 8).tobytes() == data and f.read() == b""
+            f.seek(0)
+            with pytest.raises(R.RestartError, match="expected"):
+                R.read_real_record(f, total // 8 + 1)
         # markers: leading negative while more follow, trailing negative after the first
         nsub = max(1, -(-total // lim))
         pos, heads, tails = 0, [], []
@@ -148,6 +154,10 @@ def test_state_files_round_trip_and_reject(tmp_path, monkeypatch):
     assert np.array_equal(R.read_flow2d(str(tmp_path / "flow2d.dat"), nx, ny), W)
     h2, b2 = R.read_vdf2d(str(tmp_path / "vdf2d.dat"), nx, ny, nv)
     assert np.array_equal(h2, h) and np.array_equal(b2, b)
+    # velocity planes (the device's plane transpose) write the same bytes
+    R.write_vdf2d(str(tmp_path / "vdf2d_planes.dat"), None, None, nx, ny, planes=np.stack([h.T, b.T]))
+    assert (tmp_path / "vdf2d_planes.dat").read_bytes() == (tmp_path / "vdf2d.dat").read_bytes()
+    assert np.array_equal(R.read_vdf2d_planes(str(tmp_path / "vdf2d.dat"), nx, ny, nv), np.stack([h.T, b.T]))
     with pytest.raises(R.RestartError, match="flow2d.dat does not conform"):
         R.read_flow2d(str(tmp_path / "flow2d.dat"), nx, ny + 1)
     with pytest.raises(R.RestartError, match="vdf2d.dat does not conform"):

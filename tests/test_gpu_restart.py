@@ -151,3 +151,61 @@ def test_mass_statistics_and_ledger():
     g.cold_start()
     assert math.isnan(g.mass_target()[0]) and math.isnan(g.references()[0])
     g.close()
+
+
+def test_solve_loop_writes_the_reference_files(tmp_path, release):
+    """solve_ugks with an output directory leaves what the reference's run
+    leaves after Implicit_MaxIter = 2 on the shipped case: one
+    sparse_residual.csv / consecutive_states.csv row per accepted state with
+    that state's residual norms, ResidualImplicit.plt, R5_status.txt
+    ("2 2 0 F iteration_limit", as the release run) and a complete restart
+    that loads back into the final state bit for bit."""
+    import ugks2d_high_order_amd as U
+    case = config.shipped_case()
+    tab = (release["u"], release["v"], release["w"])
+    print("solve loop with output: shipped case, 2 iterations", flush=True)
+    g = U.UGKS(case, *tab)
+    g.cold_start()
+    out = tmp_path / "run"
+    out.mkdir()
+    s, hist = U.solve_ugks(case, max_iter=2, solver=g, output=str(out))
+    lines = (out / "sparse_residual.csv").read_text().splitlines()
+    assert lines[0] == R.SPARSE_HEADER and len(lines) == 3
+    for row, h in zip(lines[1:], hist):
+        f = row.split(",")
+        assert f[:3] == [str(h["iter"]), str(h["iter"]), "accepted_convergence_verification"]
+        v = np.array([float(t) for t in f[3:]])
+        sci = v[12:34]
+        assert np.array_equal(np.delete(sci, 18), np.delete(h["norms"][:22], 18))
+        assert np.array_equal(v[34:50], h["walls"])
+    assert (out / "R5_status.txt").read_text().splitlines()[1] == "2 2 0 F iteration_limit"
+    assert len((out / "ResidualImplicit.plt").read_text().splitlines()) == 3
+    assert (out / "consecutive_states.csv").read_text().splitlines()[1:] == ["1,F,0", "2,F,0"]
+    W, h, b = g.state()
+    g2 = U.UGKS(case, *tab)
+    assert R.load_restart(g2, str(out)) == (2, 0)
+    W2, h2, b2 = g2.state()
+    assert np.array_equal(W, W2) and np.array_equal(h, h2) and np.array_equal(b, b2)
+    g.close(); g2.close()
+
+
+def test_state_planes_ragged():
+    """ugks_get_state_planes / ugks_set_state_planes on 135 cells (not a
+    multiple of the 64-cell tile) x 256 velocities: the planes are the exact
+    transpose of the [cell, Nvel] state, in both directions."""
+    import ugks2d_high_order_amd as U
+    case = config.small_case(nx=9, ny=15)
+    u, v, w = velocity.table(case.table, case.table_scale)
+    print("state planes: 9x15 cells x 16x16 velocities", flush=True)
+    g = U.UGKS(case, u, v, w)
+    rng = np.random.default_rng(11)
+    nc, nv = 135, 256
+    W, h, b = rng.normal(size=(nc, 4)), rng.normal(size=(nc, nv)), rng.normal(size=(nc, nv))
+    g.set_state(W, h, b)
+    W2, P = g.state_planes()
+    assert np.array_equal(W2, W) and np.array_equal(P[0], h.T) and np.array_equal(P[1], b.T)
+    W3, P3 = rng.normal(size=(nc, 4)), rng.normal(size=(2, nv, nc))
+    g.set_state_planes(W3, P3)
+    W4, h4, b4 = g.state()
+    assert np.array_equal(W4, W3) and np.array_equal(h4, P3[0].T) and np.array_equal(b4, P3[1].T)
+    g.close()

@@ -28,7 +28,8 @@ EXPORTS = [
     "ugks_batch_run_eager", "ugks_batch_set_shape", "ugks_batch_autotune", "ugks_wall_fluxes",
     "ugks_cn_adjoint_device", "ugks_cn_adjoint_segment", "ugks_set_physics", "ugks_batch_cn_forward",
     "ugks_batch_cn_adjoint_segment", "ugks_batch_set_physics", "ugks_set_mass_target", "ugks_get_mass_target",
-    "ugks_inherit_references", "ugks_get_references", "ugks_mass_statistics",
+    "ugks_inherit_references", "ugks_get_references", "ugks_mass_statistics", "ugks_get_state_planes",
+    "ugks_set_state_planes",
 ]
 
 FAILURE = {1: "non-finite value", 2: "moment/flux gate", 3: "inadmissible candidate", 4: "policy reject",
@@ -72,6 +73,8 @@ def load_library(path: str = LIB_PATH):
     L.ugks_get_state.argtypes = [C.c_void_p, _D, _D, _D]
     L.ugks_set_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ugks_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ugks_get_state_planes.argtypes = [C.c_void_p, _D, _D]
+    L.ugks_set_state_planes.argtypes = [C.c_void_p, _D, _D]
     L.ugks_cold_start.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.ugks_evolution.argtypes = [C.c_void_p] + [_D] * 8
     L.ugks_norms.argtypes = [C.c_void_p, _D, _D]
@@ -197,6 +200,18 @@ class UGKS:
         W = np.zeros((self.ncell, 4)); h = np.zeros((self.ncell, self.nvel)); b = np.zeros((self.ncell, self.nvel))
         _check(load_library().ugks_get_state(self.handle, _ptr(W), _ptr(h), _ptr(b)))
         return W, h, b
+
+    def state_planes(self):
+        """(W[cell, 4], planes[2, Nvel, cell]): the distributions as velocity
+        planes, h then b (the order of vdf2d.dat), transposed on the device."""
+        W = np.empty((self.ncell, 4)); planes = np.empty((2, self.nvel, self.ncell))
+        _check(load_library().ugks_get_state_planes(self.handle, _ptr(W), _ptr(planes)))
+        return W, planes
+
+    def set_state_planes(self, W, planes):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.ncell, 4)
+        planes = np.ascontiguousarray(planes, dtype=np.float64).reshape(2, self.nvel, self.ncell)
+        _check(load_library().ugks_set_state_planes(self.handle, _ptr(W), _ptr(planes)))
 
     def evolution(self, outputs: bool = True):
         """Operator on the current state; returns dt, rates and face fluxes."""
@@ -471,7 +486,9 @@ class Batch:
         return (out[0], out[1]), table
 
 
-def solve_ugks(case, max_iter: int | None = None, streak_needed: int = 5, callback=None, solver: UGKS | None = None):
+def solve_ugks(case, max_iter: int | None = None, streak_needed: int = 5, callback=None, solver: UGKS | None = None,
+               output: str | None = None, restart_every: int = 0, origin: int = 0, epoch: int = 0,
+               np_y: int = 1):
     """Pseudo-time loop of the HighOrderImplicit entry (Solve_UGKS with the
     closed-Couette R5/R6 acceptance, src/mod_Solve.f90:84-293,
     src/mod_R5Observe.f90:287-320): each iteration evaluates the operator of
@@ -479,27 +496,63 @@ def solve_ugks(case, max_iter: int | None = None, streak_needed: int = 5, callba
     once `streak_needed` consecutive accepted states pass the gate
     (macro residuals and h/b norms <= Implicit_ResidualTol), and otherwise
     applies one implicit update.  The operator of a state is computed once
-    and serves both its verification and the next update."""
+    and serves both its verification and the next update.
+
+    With `output` (a directory) the run also leaves the reference's files
+    there: per accepted state a sparse_residual.csv and a consecutive_states.csv
+    row (the gate then also requires the moment error of MassStatistics <=
+    Implicit_MomentTol, as WriteMonitorRows does) and a ResidualImplicit.plt row
+    per update; a complete restart every `restart_every` cumulative updates
+    and at the end (closed-mass companions for 1 x np_y ranks); R5_status.txt
+    at the end.  `origin` and `epoch` are the cumulative update count and the
+    residual epoch of a loaded restart (restart.load_restart returns both)."""
+    from . import restart as R
     s = solver or UGKS(case)
     if solver is None:
         s.cold_start()
     n = case.max_iter if max_iter is None else max_iter
     streak, it = 0, 0
     history = []
-    while it < n:
+    last = None            # diag16 of the last update
+    passed = False
+    reason = "iteration_limit"
+    while True:
+        if it == n and output is None:
+            break
         s.evolution(outputs=False)
         norms, walls = s.norms()
         if it > 0:
-            ok = bool(np.all(norms[0:4] <= case.residual_tol) and norms[12] <= case.residual_tol
-                      and norms[13] <= case.residual_tol)
-            streak = streak + 1 if ok else 0
+            if output is not None:
+                totals, metrics, _ok = s.mass_statistics()
+                passed = R.state_passes(norms, metrics[0], case.residual_tol, case.moment_tol)
+            else:
+                passed = bool(np.all(norms[0:4] <= case.residual_tol) and norms[12] <= case.residual_tol
+                              and norms[13] <= case.residual_tol)
+            streak = streak + 1 if passed else 0
             history.append(dict(iter=it, norms=norms, walls=walls, streak=streak))
+            if output is not None:
+                cum = origin + it
+                R.append_sparse_residual(output, R.sparse_residual_row(
+                    it, cum, "accepted_convergence_verification", norms, walls, totals, metrics, last[2],
+                    case.residual_tol, case.moment_tol, rescale=case.mass_rescale))
+                R.append_consecutive_state(output, cum, passed, streak)
+                R.append_residual_plt(output, R.residual_plt_row(
+                    cum, last[7], case.ncfl, last[0], norms, metrics[0], c_minus_1=last[2],
+                    rescale=case.mass_rescale), first=(it == 1 and origin == 0))
+                if restart_every and cum % restart_every == 0:
+                    R.save_restart(s, output, step=cum, epoch=epoch, np_y=np_y)
             if callback:
                 callback(it, norms, walls)
             if streak >= streak_needed:
+                reason = "converged"
                 break
-        acc, d = s.implicit_step()
+        if it == n:   # the last accepted state has been evaluated (and recorded)
+            break
+        acc, last = s.implicit_step()
         if not acc:
             raise UGKSError(4, "update rejected")
         it += 1
+    if output is not None:
+        R.save_restart(s, output, step=origin + it, epoch=epoch, np_y=np_y)
+        R.write_r5_status(output, it, origin + it, streak, passed, reason)
     return s, history

@@ -30,6 +30,8 @@ extern "C" hipError_t ugks_launch_observables(const LaunchSet* L, double* out, h
 extern "C" hipError_t ugks_launch_lo_step(const LaunchSet* L, int wg, hipStream_t s, Prof* P);
 extern "C" hipError_t ugks_launch_lo_cold_start(const LaunchSet* L, double rho, double ux, double tn, hipStream_t s);
 extern "C" hipError_t ugks_launch_mass_stats(const LaunchSet* L, double* part, double moment_tol, hipStream_t s);
+extern "C" hipError_t ugks_launch_planes(double* f, double* planes, int nc, int nvel, int k0, int nk, int to_planes,
+                                         hipStream_t s);
 
 namespace {
 
@@ -75,6 +77,7 @@ struct Instance {
     int seg_cap = 0;
     int lo_wg = 256;              // workgroup size of the low-order Explicit kernels
     double* mstat = nullptr;      // [cell][12] partials of the mass statistics (first use)
+    double* pstage = nullptr;     // [64][cell] velocity planes in transit (first use)
 };
 
 // Polynomial coefficients (degrees 0..n-1) whose averages over the unit cells
@@ -369,6 +372,50 @@ extern "C" int ugks_set_state_device(void* handle, const double* W, const double
 extern "C" int ugks_get_state_device(void* handle, double* W, double* h, double* b) {
     if (!handle) return fail(E_ARG, "null handle");
     return get_state_impl((Instance*)handle, W, h, b, hipMemcpyDeviceToDevice);
+}
+
+// The distributions as velocity planes ([2][Nvel][cell]: h then b, the
+// layout of vdf2d.dat), moved through a 64-plane device stage so that no
+// host-side transpose is needed.
+static int planes_impl(Instance* I, double* W, double* planes, bool to_host) {
+    Dev& D = I->D;
+    const int nc = D.ncell, nv = D.nvel;
+    int rc;
+    if (!I->pstage && (rc = dmalloc(I, &I->pstage, (size_t)nc * 64))) return rc;
+    if (to_host) {
+        HIPCHK(hipMemcpyAsync(W, D.W, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost, I->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(D.W, W, sizeof(double) * 4 * nc, hipMemcpyHostToDevice, I->stream));
+    }
+    for (int q = 0; q < 2; ++q) {
+        double* f = q ? D.b : D.h;
+        for (int k0 = 0; k0 < nv; k0 += 64) {
+            const int nk = nv - k0 < 64 ? nv - k0 : 64;
+            double* host = planes + ((size_t)q * nv + k0) * nc;
+            const size_t bytes = sizeof(double) * (size_t)nk * nc;
+            if (to_host) {
+                HIPCHK(ugks_launch_planes(f, I->pstage, nc, nv, k0, nk, 1, I->stream));
+                HIPCHK(hipMemcpyAsync(host, I->pstage, bytes, hipMemcpyDeviceToHost, I->stream));
+            } else {
+                HIPCHK(hipMemcpyAsync(I->pstage, host, bytes, hipMemcpyHostToDevice, I->stream));
+                HIPCHK(ugks_launch_planes(f, I->pstage, nc, nv, k0, nk, 0, I->stream));
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(I->stream));
+    if (to_host) return E_OK;
+    I->operator_fresh = false;
+    return reset_cn(I);
+}
+
+extern "C" int ugks_get_state_planes(void* handle, double* W, double* planes) {
+    if (!handle || !W || !planes) return fail(E_ARG, "null argument");
+    return planes_impl((Instance*)handle, W, planes, true);
+}
+
+extern "C" int ugks_set_state_planes(void* handle, const double* W, const double* planes) {
+    if (!handle || !W || !planes) return fail(E_ARG, "null argument");
+    return planes_impl((Instance*)handle, const_cast<double*>(W), const_cast<double*>(planes), false);
 }
 
 extern "C" int ugks_cold_start(void* handle, double rho, double ux, double t_native) {

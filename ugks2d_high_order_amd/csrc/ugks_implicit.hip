@@ -1006,6 +1006,51 @@ extern "C" hipError_t ugks_launch_mass_stats(const LaunchSet* L, double* part, d
     return hipGetLastError();
 }
 
+// Velocity planes <-> the [cell][Nvel] state, nk <= 64 planes from plane k0
+// at a time: planes[r][c] = f[c][k0 + r] (the layout of vdf2d.dat, one plane
+// per velocity with the cells in file order).  A 64-cell x 64-plane tile is
+// staged in LDS so that both the [cell][Nvel] rows (512 B per cell) and the
+// plane rows (512 B per plane) are read and written contiguously.
+template <bool TO_PLANES>
+__global__ __launch_bounds__(256) void k_planes(double* __restrict__ f, double* __restrict__ planes, int nc,
+                                                int nvel, int k0, int nk) {
+    __shared__ double t[64][65];
+    const int c0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (TO_PLANES) {
+        for (int r = ty; r < 64; r += 4) {
+            const int c = c0 + r;
+            if (c < nc && tx < nk) t[r][tx] = f[(size_t)c * nvel + k0 + tx];
+        }
+        __syncthreads();
+        for (int r = ty; r < nk; r += 4) {
+            const int c = c0 + tx;
+            if (c < nc) planes[(size_t)r * nc + c] = t[tx][r];
+        }
+    } else {
+        for (int r = ty; r < nk; r += 4) {
+            const int c = c0 + tx;
+            if (c < nc) t[tx][r] = planes[(size_t)r * nc + c];
+        }
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            const int c = c0 + r;
+            if (c < nc && tx < nk) f[(size_t)c * nvel + k0 + tx] = t[r][tx];
+        }
+    }
+}
+
+extern "C" hipError_t ugks_launch_planes(double* f, double* planes, int nc, int nvel, int k0, int nk, int to_planes,
+                                         hipStream_t s) {
+    if (nk < 1 || nk > 64 || k0 < 0 || k0 + nk > nvel || nc < 1) return hipErrorInvalidValue;
+    const dim3 grid((nc + 63) / 64);
+    if (to_planes)
+        hipLaunchKernelGGL(k_planes<true>, grid, dim3(256), 0, s, f, planes, nc, nvel, k0, nk);
+    else
+        hipLaunchKernelGGL(k_planes<false>, grid, dim3(256), 0, s, f, planes, nc, nvel, k0, nk);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t ugks_launch_wall_reflection(const LaunchSet* L, hipStream_t s, Prof* P) {
     UGKS_TIMED(P, K_WALL_REFLECTION, s,
                hipLaunchKernelGGL(k_wall_reflection, dim3(1, 1, L->n), dim3(256), 0, s, L->dev));

@@ -45,6 +45,20 @@ module ugks_hip
       real(c_double), intent(out) :: W(*), h(*), b(*)
     end function ugks_get_state
 
+    ! the state with h and b as velocity planes, planes(ncell, Nvel, 2): the
+    ! third record of vdf2d.dat, transposed on the device
+    integer(c_int) function ugks_get_state_planes(handle, W, planes) bind(C, name="ugks_get_state_planes")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: W(*), planes(*)
+    end function ugks_get_state_planes
+
+    integer(c_int) function ugks_set_state_planes(handle, W, planes) bind(C, name="ugks_set_state_planes")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: handle
+      real(c_double), intent(in) :: W(*), planes(*)
+    end function ugks_set_state_planes
+
     integer(c_int) function ugks_cold_start(handle, rho, ux, t_native) bind(C, name="ugks_cold_start")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: handle

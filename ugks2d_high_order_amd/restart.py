@@ -125,6 +125,36 @@ def read_record(f) -> bytes:
             return b"".join(chunks) if len(chunks) > 1 else chunks[0]
 
 
+def read_real_record(f, n: int):
+    """Read one sequential record that holds n doubles straight into an array
+    (the subrecords are read in place, with no joined copy)."""
+    out = np.empty(n, dtype="<f8")
+    buf = memoryview(out).cast("B")
+    pos, first = 0, True
+    while True:
+        head = f.read(4)
+        if len(head) < 4:
+            raise RestartError("unexpected end of file in a record marker")
+        m = struct.unpack("<i", head)[0]
+        if pos + abs(m) > len(buf):
+            raise RestartError(f"record holds more than {8 * n} bytes, expected {8 * n}")
+        if f.readinto(buf[pos:pos + abs(m)]) != abs(m):
+            raise RestartError("truncated record")
+        pos += abs(m)
+        tail = f.read(4)
+        if len(tail) < 4:
+            raise RestartError("truncated record")
+        t = struct.unpack("<i", tail)[0]
+        if abs(t) != abs(m) or (t < 0) == first:
+            raise RestartError("inconsistent record markers")
+        first = False
+        if m >= 0:
+            break
+    if pos != len(buf):
+        raise RestartError(f"record holds {pos} bytes, expected {8 * n}")
+    return out
+
+
 def read_records(path: str):
     with open(path, "rb") as f:
         size = os.fstat(f.fileno()).st_size
@@ -215,22 +245,28 @@ def read_flow2d(path: str, nx: int, ny: int):
     return np.ascontiguousarray(a.transpose(1, 2, 0)).reshape(nx * ny, 4)
 
 
-def write_vdf2d(path: str, h, b, nx: int, ny: int):
-    """vdf2d.dat of the structured-quadrature branch; h and b are streamed
-    64 velocity planes at a time (no full transposed copy)."""
-    nv = h.shape[-1]
-    h = np.asarray(h).reshape(ny * nx, nv)
-    b = np.asarray(b).reshape(ny * nx, nv)
-    # velocity planes in file order, transposed 64 at a time
-    planes = (np.ascontiguousarray(a[:, k0:k0 + 64].T, dtype="<f8") for a in (h, b) for k0 in range(0, nv, 64))
+def write_vdf2d(path: str, h, b, nx: int, ny: int, planes=None):
+    """vdf2d.dat of the structured-quadrature branch.  Either `planes`
+    ([2, Nvel, cell]: h then b as velocity planes, the file's order, e.g. from
+    UGKS.state_planes) is written as is, or h and b ([cell, Nvel]) are
+    transposed 64 velocity planes at a time (no full transposed copy)."""
+    if planes is not None:
+        planes = np.asarray(planes)
+        nv = planes.shape[1]
+        parts = [np.ascontiguousarray(planes, dtype="<f8")]
+    else:
+        nv = h.shape[-1]
+        h = np.asarray(h).reshape(ny * nx, nv)
+        b = np.asarray(b).reshape(ny * nx, nv)
+        parts = (np.ascontiguousarray(a[:, k0:k0 + 64].T, dtype="<f8") for a in (h, b) for k0 in range(0, nv, 64))
     with open(path, "wb") as f:
         write_record(f, [np.array([1], dtype="<i4")])
         write_record(f, [np.array([nx, ny, 2 * nv], dtype="<i4")])
-        write_record(f, planes, total=8 * 2 * nv * nx * ny)
+        write_record(f, parts, total=8 * 2 * nv * nx * ny)
 
 
-def read_vdf2d(path: str, nx: int, ny: int, nvel: int):
-    """(h[c, Nvel], b[c, Nvel]) of vdf2d.dat."""
+def read_vdf2d_planes(path: str, nx: int, ny: int, nvel: int):
+    """The third record of vdf2d.dat as planes[2, Nvel, cell] (h then b)."""
     if not os.path.exists(path):
         raise RestartError("Restart error: file vdf2d.dat does not exist.")
     with open(path, "rb") as f:
@@ -238,7 +274,12 @@ def read_vdf2d(path: str, nx: int, ny: int, nvel: int):
         n0 = _ints(read_record(f))
         if tuple(n0[:3]) != (nx, ny, 2 * nvel):
             raise RestartError(f"Restart error: data in vdf2d.dat does not conform. (nx0, ny0, nvar0) = {n0[:3]}")
-        a = _reals(read_record(f), 2 * nvel * nx * ny).reshape(2, nvel, nx * ny)
+        return read_real_record(f, 2 * nvel * nx * ny).reshape(2, nvel, nx * ny)
+
+
+def read_vdf2d(path: str, nx: int, ny: int, nvel: int):
+    """(h[c, Nvel], b[c, Nvel]) of vdf2d.dat."""
+    a = read_vdf2d_planes(path, nx, ny, nvel)
     return np.ascontiguousarray(a[0].T), np.ascontiguousarray(a[1].T)
 
 
@@ -327,37 +368,50 @@ def blocks(nx_all: int, ny_all: int, np_x: int, np_y: int):
     return out
 
 
-def _halo_block(field, nx_all, ny_all, oi, oj, ni, nj):
-    """Cells (i, j) = 1-LAP..ni+LAP x 1-LAP..nj+LAP of a block, as the
-    reference holds them after its halo exchanges: x wraps periodically
-    (BC 10/10 and the corner exchange), rows beyond the walls are zero
-    (the wall-exterior slots are zero-initialised and never written,
-    src/mod_InitializeFlow.f90:116-125)."""
-    nf = field.shape[-1]
-    g = field.reshape(ny_all, nx_all, nf)
-    J = np.arange(oj - LAP, oj + nj + LAP) - 1          # global 0-based rows
-    I = (np.arange(oi - LAP, oi + ni + LAP) - 1) % nx_all
-    out = np.zeros((len(J), len(I), nf))
-    inside = (J >= 0) & (J < ny_all)
-    out[inside] = g[J[inside]][:, I]
+def _runs(idx):
+    """(position, start, stop) of the runs of consecutive values in idx."""
+    out, p0 = [], 0
+    for p in range(1, len(idx) + 1):
+        if p == len(idx) or idx[p] != idx[p - 1] + 1:
+            out.append((p0, int(idx[p0]), int(idx[p - 1]) + 1))
+            p0 = p
     return out
 
 
 def write_mass_restart(directory: str, W, h, b, identity: RunIdentity, m0: float, ledger, step: int):
-    """closed_mass_rank<r>.bin of every rank of identity's decomposition."""
+    """closed_mass_rank<r>.bin of every rank of identity's decomposition.
+    Each holds the cells (i, j) = 1-LAP..ni+LAP x 1-LAP..nj+LAP of its block
+    as the reference holds them after its halo exchanges: x wraps
+    periodically (BC 10/10 and the corner exchange), rows beyond the walls are
+    zero (the wall-exterior slots are zero-initialised and never written,
+    src/mod_InitializeFlow.f90:116-125).  The cells are streamed one halo row
+    at a time through one reused row buffer."""
     nx_all, ny_all = identity.nx, identity.ny
     nv = h.shape[-1]
+    Wg = np.asarray(W, dtype=np.float64).reshape(ny_all, nx_all, 4)
+    hg = np.asarray(h, dtype=np.float64).reshape(ny_all, nx_all, nv)
+    bg = np.asarray(b, dtype=np.float64).reshape(ny_all, nx_all, nv)
     ledger = np.asarray(ledger, dtype="<f8").reshape(4)
     for r, oi, oj, ni, nj in blocks(nx_all, ny_all, identity.np_x, identity.np_y):
-        cells = np.concatenate([_halo_block(np.asarray(W).reshape(-1, 4), nx_all, ny_all, oi, oj, ni, nj),
-                                _halo_block(np.asarray(h).reshape(-1, nv), nx_all, ny_all, oi, oj, ni, nj),
-                                _halo_block(np.asarray(b).reshape(-1, nv), nx_all, ny_all, oi, oj, ni, nj)], -1)
+        I = (np.arange(oi - LAP, oi + ni + LAP) - 1) % nx_all
+        runs = _runs(I)
+        row = np.empty((ni + 2 * LAP, 4 + 2 * nv), dtype="<f8")
+        zero = np.zeros_like(row)
         with open(os.path.join(directory, f"closed_mass_rank{r}.bin"), "wb") as f:
             f.write(np.array([MASS_VERSION, ni, nj, nv, LAP, step], dtype="<i4").tobytes())
             f.write(np.array([m0], dtype="<f8").tobytes())
             f.write(ledger.tobytes())
             f.write(identity.vector().astype("<f8").tobytes())
-            f.write(np.ascontiguousarray(cells, dtype="<f8").tobytes())
+            for j in range(oj - LAP - 1, oj + nj + LAP - 1):      # global 0-based rows
+                if not 0 <= j < ny_all:
+                    f.write(memoryview(zero).cast("B"))
+                    continue
+                for p, a, e in runs:
+                    q = p + e - a
+                    row[p:q, 0:4] = Wg[j, a:e]
+                    row[p:q, 4:4 + nv] = hg[j, a:e]
+                    row[p:q, 4 + nv:] = bg[j, a:e]
+                f.write(memoryview(row).cast("B"))
 
 
 @dataclasses.dataclass
@@ -384,20 +438,23 @@ def read_mass_restart(directory: str, W, h, b, identity: RunIdentity) -> MassRes
         path = os.path.join(directory, f"closed_mass_rank{r}.bin")
         if not os.path.exists(path):
             raise RestartError("missing, mismatched, or invalid M0-bearing restart; refusing to redefine M0")
-        raw = np.fromfile(path, dtype=np.uint8)
         ncell = (ni + 2 * LAP) * (nj + 2 * LAP)
         need = hdr_n + 8 * nid + 8 * ncell * (4 + 2 * nv)
-        if raw.size != need:
-            raise RestartError(f"{os.path.basename(path)}: {raw.size} bytes, expected {need}")
-        head = np.frombuffer(raw[:24].tobytes(), dtype="<i4")
-        m0 = float(np.frombuffer(raw[24:32].tobytes(), dtype="<f8")[0])
-        ledger = np.frombuffer(raw[32:64].tobytes(), dtype="<f8").copy()
-        ident = np.frombuffer(raw[64:64 + 8 * nid].tobytes(), dtype="<f8")
+        size = os.path.getsize(path)
+        if size != need:
+            raise RestartError(f"{os.path.basename(path)}: {size} bytes, expected {need}")
+        with open(path, "rb") as f:
+            raw = f.read(hdr_n + 8 * nid)
+            cells = np.fromfile(f, dtype="<f8")
+        head = np.frombuffer(raw[:24], dtype="<i4")
+        m0 = float(np.frombuffer(raw[24:32], dtype="<f8")[0])
+        ledger = np.frombuffer(raw[32:64], dtype="<f8").copy()
+        ident = np.frombuffer(raw[64:64 + 8 * nid], dtype="<f8")
         ok = (tuple(head[:5]) == (MASS_VERSION, ni, nj, nv, LAP) and np.array_equal(ident, want)
               and math.isfinite(m0) and m0 > 0.0 and np.all(np.isfinite(ledger)))
         if not ok:
             raise RestartError("missing, mismatched, or invalid M0-bearing restart; refusing to redefine M0")
-        cells = np.frombuffer(raw[64 + 8 * nid:].tobytes(), dtype="<f8").reshape(nj + 2 * LAP, ni + 2 * LAP, 4 + 2 * nv)
+        cells = cells.reshape(nj + 2 * LAP, ni + 2 * LAP, 4 + 2 * nv)
         inner = cells[LAP:LAP + nj, LAP:LAP + ni]
         sl = (slice(oj - 1, oj - 1 + nj), slice(oi - 1, oi - 1 + ni))
         if not (np.array_equal(inner[..., :4], Wg[sl]) and np.array_equal(inner[..., 4:4 + nv], hg[sl])
@@ -509,7 +566,7 @@ def read_residual_reference(path: str, residual_tol: float | None = None, moment
 # ---------------------------------------------------------------------------
 def write_restart(directory: str, case, W, h, b, u, w, m0: float, ledger, ref_h: float, ref_b: float,
                   step: int, epoch: int = 0, np_x: int = 1, np_y: int = 1, nodes=None,
-                  identity: RunIdentity | None = None):
+                  identity: RunIdentity | None = None, planes=None):
     """Write the complete restart of a HighOrderImplicit run after `step`
     cumulative updates (Output_restart_only, src/mod_Output.f90:38-56):
     Mesh2d.dat, flow2d.dat, vdf2d.dat, grid2d.grd, closed_mass_rank*.bin for
@@ -520,7 +577,7 @@ def write_restart(directory: str, case, W, h, b, u, w, m0: float, ledger, ref_h:
     ident = identity or run_identity(case, u, w, np_x=np_x, np_y=np_y)
     write_mesh2d(os.path.join(directory, "Mesh2d.dat"), *nodes)
     write_flow2d(os.path.join(directory, "flow2d.dat"), W, case.nx, case.ny)
-    write_vdf2d(os.path.join(directory, "vdf2d.dat"), h, b, case.nx, case.ny)
+    write_vdf2d(os.path.join(directory, "vdf2d.dat"), h, b, case.nx, case.ny, planes=planes)
     write_grid2d(os.path.join(directory, "grid2d.grd"), *nodes)
     write_mass_restart(directory, W, h, b, ident, m0, ledger, step)
     write_residual_reference(os.path.join(directory, "residual_reference.in"), epoch, ref_h, ref_b,
@@ -531,8 +588,11 @@ def write_restart(directory: str, case, W, h, b, u, w, m0: float, ledger, ref_h:
 def save_restart(solver, directory: str, step: int, epoch: int = 0, np_x: int = 1, np_y: int = 1, nodes=None,
                  identity: RunIdentity | None = None):
     """write_restart of a GPU instance's current state, mass target, ledger
-    and residual references (the operator must have been evaluated once)."""
-    W, h, b = solver.state()
+    and residual references (the operator must have been evaluated once).
+    vdf2d.dat is written from the device's velocity planes (state_planes), the
+    companions from the [cell, Nvel] state."""
+    W, planes = solver.state_planes()
+    _, h, b = solver.state()
     m0, ledger = solver.mass_target()
     if not math.isfinite(m0):
         m0 = float(solver.mass_statistics()[0][0])
@@ -540,7 +600,7 @@ def save_restart(solver, directory: str, step: int, epoch: int = 0, np_x: int = 
     if not (math.isfinite(ref_h) and math.isfinite(ref_b)):
         raise RestartError("no residual references yet: evaluate the operator before saving")
     return write_restart(directory, solver.case, W, h, b, solver.u, solver.w, m0, ledger, ref_h, ref_b, step,
-                         epoch, np_x, np_y, nodes, identity)
+                         epoch, np_x, np_y, nodes, identity, planes=planes)
 
 
 def load_restart(solver, directory: str, identity: RunIdentity | None = None):
@@ -548,7 +608,13 @@ def load_restart(solver, directory: str, identity: RunIdentity | None = None):
     = 2: InitFlow_Restart + ReadMassRestart + StartR5Control): the state from
     flow2d.dat/vdf2d.dat, M0 and the ledger from the closed_mass companions
     (whose decomposition is taken from the files present), the residual epoch
-    from residual_reference.in.  Returns (cumulative step, epoch)."""
+    from residual_reference.in.  Returns (cumulative step, epoch).
+
+    The distributions go to the device as velocity planes (set_state_planes,
+    transposed there), and the companions are checked against the [cell, Nvel]
+    state read back from the device.  After a RestartError the instance holds
+    the files' state but not their mass target or references: cold-start it
+    or load a valid restart before stepping."""
     case = solver.case
     mesh = os.path.join(directory, "Mesh2d.dat")
     if os.path.exists(mesh):
@@ -556,7 +622,7 @@ def load_restart(solver, directory: str, identity: RunIdentity | None = None):
         if (nx, ny) != (case.nx, case.ny) or abs(dx - case.dx) > 1e-14 * dx or abs(dy - case.dy) > 1e-14 * dy:
             raise RestartError("Mesh2d.dat does not match the case")
     W = read_flow2d(os.path.join(directory, "flow2d.dat"), case.nx, case.ny)
-    h, b = read_vdf2d(os.path.join(directory, "vdf2d.dat"), case.nx, case.ny, case.nu * case.nv)
+    planes = read_vdf2d_planes(os.path.join(directory, "vdf2d.dat"), case.nx, case.ny, case.nu * case.nv)
     if identity is None:
         ranks = mass_restart_ranks(directory)
         if not ranks:
@@ -565,10 +631,12 @@ def load_restart(solver, directory: str, identity: RunIdentity | None = None):
             f.seek(64 + 8 * 2)
             np_x, np_y = (int(v) for v in np.frombuffer(f.read(16), dtype="<f8"))
         identity = run_identity(case, solver.u, solver.w, np_x=np_x, np_y=np_y)
+    solver.set_state_planes(W, planes)
+    del planes
+    _, h, b = solver.state()
     mr = read_mass_restart(directory, W, h, b, identity)
     epoch, ref_h, ref_b, _, _ = read_residual_reference(os.path.join(directory, "residual_reference.in"),
                                                         case.residual_tol, case.moment_tol, origin=mr.step)
-    solver.set_state(W, h, b)
     solver.set_mass_target(mr.m0, mr.ledger)
     solver.inherit_references(ref_h, ref_b)
     return mr.step, epoch
@@ -653,7 +721,10 @@ def residual_plt_row(it: int, dt: float, ncfl: float, alpha: float, norms24, mom
 
 
 def append_residual_plt(directory: str, row: str, first: bool):
+    """Append a row; `first` starts a new file (the reference replaces it
+    on the first row of a run), a missing file also gets the header."""
     path = os.path.join(directory, "ResidualImplicit.plt")
+    first = first or not os.path.exists(path)
     with open(path, "w" if first else "a") as f:
         if first:
             f.write(RESIDUAL_PLT_HEADER + "\n")
