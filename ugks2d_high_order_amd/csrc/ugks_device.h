@@ -55,6 +55,14 @@ struct Consts {
 // ---------------------------------------------------------------------------
 #ifndef UGKS_EMU_REDUCTIONS
 // Lane moves through the DPP crossbar, on the two 32-bit halves of a double.
+// Workgroups are dispatched round-robin over the 8 XCDs.  Map dispatch slot b
+// of n to a logical index so that each XCD gets a contiguous range of logical
+// indices, walked in order: neighbouring stencils then share that XCD's L2.
+__device__ inline int xcd_contiguous(int b, int n) {
+    const int X = 8, q = n / X, r = n % X, x = b % X, t = b / X;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + t;
+}
+
 template <int CTRL>
 __device__ inline double dpp_f64(double x) {
     int lo = __double2loint(x), hi = __double2hiint(x);

@@ -29,13 +29,6 @@ namespace {
 constexpr double LO_TINY = 1.17549435082228750797e-38;   // tiny(0.0_real32), the limiter guard
 constexpr int LO_MAXAX = 64;                              // largest Nu, Nv of the tables
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
-// contiguous range of faces/cells so neighbouring stencils share its L2.
-__device__ inline int xcd_contiguous(int b, int n) {
-    const int X = 8, q = n / X, r = n % X, x = b % X, t = b / X;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + t;
-}
-
 // Interp_VanLeer with the cell-size divisions as reciprocal products
 __device__ inline double vanleer(double fl, double fc, double fr, double idl, double idr) {
     double sl = (fc - fl) * idl, sr = (fr - fc) * idr;

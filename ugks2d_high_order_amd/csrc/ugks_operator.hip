@@ -89,13 +89,26 @@ __global__ void __launch_bounds__(256) k_wrecon(const Dev* __restrict__ DS, int 
         for (int b = 0; b < 3; ++b) ov[a * 3 + b] = vol[a][b];
 }
 
+// Velocity chunk (64 velocities x 2 fields) and cell of a reconstruction
+// workgroup on the (chunks, cells) grid.  Each XCD walks a contiguous range
+// of chunk-major indices: one chunk over all cells (row-major) at a time, so
+// the five-row column stencil of that chunk stays in the XCD's L2 instead of
+// the stencils of eight interleaved chunks.
+__device__ inline void recon_block(int* chunk, int* cell) {
+    const int b = blockIdx.x + gridDim.x * blockIdx.y;
+    const int l = xcd_contiguous(b, gridDim.x * gridDim.y);
+    *chunk = l / gridDim.y;
+    *cell = l % gridDim.y;
+}
+
 // First stage: the x and y lines of every (cell, velocity, field); two
 // lanes per (cell, velocity), the even lane for h and the odd lane for b.
 __global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk, c;
+    recon_block(&chunk, &c);
+    const int t = chunk * blockDim.x + threadIdx.x;
     const int k = t >> 1, fld = t & 1;
-    const int c = blockIdx.y;
     if (c >= D.ncell) return;
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const int i = c % nx, j = c / nx;
@@ -127,9 +140,10 @@ __global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS)
 // minimum of the pair, exchanged through DPP.
 __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     const Dev& D = DS[blockIdx.z];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk, c;
+    recon_block(&chunk, &c);
+    const int t = chunk * blockDim.x + threadIdx.x;
     const int k = t >> 1, fld = t & 1;
-    const int c = blockIdx.y;
     if (c >= D.ncell) return;
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const size_t nv = (size_t)nvel;
