@@ -50,11 +50,6 @@ struct Consts {
     double wv[5][5][5], wd[5][5][5];
 };
 
-// ---------------------------------------------------------------------------
-// reductions
-// ---------------------------------------------------------------------------
-#ifndef UGKS_EMU_REDUCTIONS
-// Lane moves through the DPP crossbar, on the two 32-bit halves of a double.
 // Workgroups are dispatched round-robin over the 8 XCDs.  Map dispatch slot b
 // of n to a logical index so that each XCD gets a contiguous range of logical
 // indices, walked in order: neighbouring stencils then share that XCD's L2.
@@ -63,6 +58,11 @@ __device__ inline int xcd_contiguous(int b, int n) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + t;
 }
 
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+#ifndef UGKS_EMU_REDUCTIONS
+// Lane moves through the DPP crossbar, on the two 32-bit halves of a double.
 template <int CTRL>
 __device__ inline double dpp_f64(double x) {
     int lo = __double2loint(x), hi = __double2hiint(x);
