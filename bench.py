@@ -72,11 +72,11 @@ EXPLICIT_BYTES_PER_CV = 4 * 8
 # (nx*ny + nx*(ny-1))*Nvel.  None where not measured.
 KERNEL_COUNTS = {
     # kernel: (flop per unit, unit, HBM bytes per launch); 64x512 cells x 64x64 velocities
-    "k_recon": (7672.9, "cv", 1.1698e+11),
-    "k_face_flux": (1650.5, "fv", 8.6004e+10),
-    "k_cell": (3250.3, "cv", 1.2728e+10),
-    "k_recon_lines": (1760.5, "cv", 2.6107e+10),
-    "k_predict": (605.0, "cv", 7.4828e+09),
+    "k_recon": (7672.9, "cv", 6.8244e+10),
+    "k_face_flux": (1650.5, "fv", 8.6003e+10),
+    "k_cell": (3250.3, "cv", 1.2768e+10),
+    "k_recon_lines": (1760.5, "cv", 2.6794e+10),
+    "k_predict": (605.0, "cv", 7.5293e+09),
 }
 # measured FP64 operations of one Explicit step (both face launches, walls, cells)
 EXPLICIT_FLOP_PER_STEP = 2.1244e11

@@ -101,6 +101,14 @@ __device__ inline void recon_block(int* chunk, int* cell) {
     *cell = l % gridDim.y;
 }
 
+// The cell of strip-ordered index r < nx*ny: strips of SW columns (16 when
+// nx is a multiple of 16, else the whole row), rows walked inside a strip.
+__device__ inline int recon_strip_cell(int r, int nx, int ny) {
+    const int sw = (nx & 15) == 0 ? 16 : nx;
+    const int strip = r / (sw * ny), rem = r - strip * sw * ny;
+    return (rem / sw) * nx + strip * sw + rem % sw;
+}
+
 // First stage: the x and y lines of every (cell, velocity, field); two
 // lanes per (cell, velocity), the even lane for h and the odd lane for b.
 __global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS) {
@@ -110,6 +118,7 @@ __global__ void __launch_bounds__(128) k_recon_lines(const Dev* __restrict__ DS)
     const int t = chunk * blockDim.x + threadIdx.x;
     const int k = t >> 1, fld = t & 1;
     if (c >= D.ncell) return;
+    c = recon_strip_cell(c, D.nx, D.ny);
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const int i = c % nx, j = c / nx;
     const double* F = fld == 0 ? D.h : D.b;
@@ -145,6 +154,7 @@ __global__ void __launch_bounds__(128) k_recon(const Dev* __restrict__ DS) {
     const int t = chunk * blockDim.x + threadIdx.x;
     const int k = t >> 1, fld = t & 1;
     if (c >= D.ncell) return;
+    c = recon_strip_cell(c, D.nx, D.ny);
     const int nx = D.nx, ny = D.ny, nvel = D.nvel;
     const size_t nv = (size_t)nvel;
     const int i = c % nx, j = c / nx;
